@@ -1,0 +1,98 @@
+/*
+ * rectv_amd.h -- C ABI of the MI355X-native rectv solver (librectv_amd.so).
+ *
+ * Plain C types only: host pointers, sizes, scalars.  Every entry point returns
+ * 0 on success and a negative status on failure; rtv_last_error() then returns
+ * a message (thread-local).  All host arrays are C-contiguous float32/float64
+ * in the reference's layouts:
+ *   g      [nz][ntheta][n]          float32   projection data
+ *   theta  [ntheta]                 float32   angles (radians)
+ *   phi    [m][2*ntheta]            float32   basis, re/im interleaved
+ *   fres   [nz][m][n][n]            float32   reconstruction
+ * Citations are to the reference nikitinvv/rectv_gpu (paths relative to its root).
+ */
+#ifndef RECTV_AMD_H
+#define RECTV_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rtv_solver rtv_solver;
+
+enum rtv_status {
+    RTV_OK = 0,
+    RTV_ERR_ARG = -1,      /* invalid argument / shape */
+    RTV_ERR_HIP = -2,      /* HIP runtime error */
+    RTV_ERR_NOMEM = -3,    /* device allocation failed */
+    RTV_ERR_STATE = -4     /* call not valid in the current state (e.g. freed) */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *rtv_last_error(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int rtv_device_count(void);
+
+/* Replaces rectv::rectv(n, ntheta, m, nz, nzp, ngpus), src/cuda/rectv.cu:5-59
+ * (SWIG ctor src/cuda/rectv.i:21).  ngpus is clamped like the reference
+ * (min(ngpus, nz/nzp), rectv.cu:9) and then to the one GPU this build drives.
+ * Fails (instead of truncating, rectv.cu:162) when nz % nzp != 0. */
+int rtv_create(int n, int ntheta, int m, int nz, int nzp, int ngpus, rtv_solver **out);
+
+/* Replaces rectv::free(), src/cuda/rectv.cu:63-91 (idempotent; never resets the
+ * device).  Also releases the handle itself. */
+int rtv_free(rtv_solver *h);
+
+/* Replaces rectv::run(fres, g, theta, phi, center, lambda0, lambda1, niter, titer),
+ * src/cuda/rectv.cu:102-252.  lambda0 is used as given (the Python layer
+ * multiplies it by lambda1 first, src/rectv_gpu/solver.py:76).  Prints the
+ * reference's per-iteration line to stderr unless RECTV_QUIET=1. */
+int rtv_run(rtv_solver *h, float *fres, const float *g, const float *theta, const float *phi,
+            float center, float lambda0, float lambda1, int niter, int titer);
+
+/* Replaces rectv::check_adjoints(res, g, theta, phi, lambda1, center),
+ * src/cuda/rectv.cu:255-289 (requires nz == nzp, as the reference states at :258).
+ * res receives 6 doubles. */
+int rtv_check_adjoints(rtv_solver *h, double *res, const float *g, const float *theta,
+                       const float *phi, float lambda1, float center);
+
+/* ---- staged form of rtv_run (same arithmetic), used to time the solver with its
+ * inputs already resident in device memory.  rtv_run == set_inputs + reset +
+ * iterate + get_result.  No reference counterpart: the reference fuses these
+ * steps inside rectv::run (rectv.cu:107-125 upload/zero, :140-246 loop, :250-251
+ * download). */
+int rtv_set_inputs(rtv_solver *h, const float *g, const float *theta, const float *phi, float center);
+int rtv_reset(rtv_solver *h);
+/* Runs niter outer iterations.  hist (may be NULL) receives niter rows of
+ * {r, s, rho used, rdiff = ||f(k+1)-f(k)||^2}. */
+int rtv_iterate(rtv_solver *h, int niter, int titer, float lambda0, float lambda1, double *hist);
+int rtv_get_result(rtv_solver *h, float *fres);
+int rtv_synchronize(rtv_solver *h);
+
+/* ---- operator entry points (parity tests).  Each applies one reference operator to
+ * all nz slices, with inputs/outputs in host memory; rtv_set_inputs must have
+ * been called (it supplies theta, phi and center). */
+/* rectv::radonapr(g, f, tau=1), src/cuda/operators.cu:6-32:  g = R_apr f. */
+int rtv_apply_radon(rtv_solver *h, const float *f, float *g);
+/* rectv::radonapradj(f, g, tau=1) into zero, src/cuda/operators.cu:34-58:  f = R_apr^* g. */
+int rtv_apply_radon_adj(rtv_solver *h, const float *g, float *f);
+/* rectv::gradient(h2, f, lambda1, iz), src/cuda/operators.cu:60-66 for chunk iz:
+ * fext [nzp+2][m][n][n] (slab 0 / nzp+1 = halos, read when the chunk has a
+ * neighbour); h2 out [nzp+1][m+1][n+1][n+1][4] (the reference's float4 order). */
+int rtv_apply_gradient(rtv_solver *h, const float *fext, int iz, float lambda1, float *h2);
+/* rectv::divergent(fn, h2, lambda1, tau), src/cuda/operators.cu:68-71:
+ * f [nzp][m][n][n] += tau * div(h2). */
+int rtv_apply_divergence(rtv_solver *h, const float *h2, float lambda1, float tau, float *f);
+
+/* ---- profiling: per-kernel HIP-event timing on the solver's stream. */
+int rtv_set_profiling(rtv_solver *h, int on);
+/* Writes "name count total_ms\n" lines into buf (NUL-terminated, truncated to len). */
+int rtv_get_profile(rtv_solver *h, char *buf, int len);
+/* Device bytes held by the solver (persistent state + scratch). */
+long long rtv_device_bytes(rtv_solver *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RECTV_AMD_H */

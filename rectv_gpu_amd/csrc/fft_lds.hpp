@@ -1,0 +1,209 @@
+// LDS-staged mixed-radix Stockham FFT executed cooperatively by one workgroup.
+//
+// T transforms of length N live in LDS at buf[t*stride + i].  Each stage reads
+// every butterfly's R inputs into registers, synchronises, applies the radix-R
+// DFT and writes the autosorted outputs back into the same buffer, so one
+// buffer of T*stride complex values is enough.  Radices 2,3,4,5,7,8; twiddles
+// come from a table tw[m] = exp(-2 pi i m / N) computed in FP64 on the host.
+// Forward = exp(-2 pi i jk/N), inverse = exp(+2 pi i jk/N), both unnormalised.
+// Requirement: blockDim.x >= T * ceil(N / E) (E = max elements held per thread).
+#pragma once
+#include "common.hpp"
+
+namespace rtv {
+
+constexpr int FFT_MAX_STAGES = 16;
+struct FftPlan {
+    int n;
+    int nst;
+    int radix[FFT_MAX_STAGES];
+    int tw_off;  // offset of this size's twiddle table in the shared table buffer
+};
+
+template <bool INV>
+__device__ inline float2 mul_mi(float2 a) {  // a * (-i) forward, a * (+i) inverse
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ inline void dft2(float2 *v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <bool INV>
+__device__ inline void dft4(float2 *v) {
+    float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    float2 s13 = cadd(v[1], v[3]), d13 = mul_mi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+}
+
+template <bool INV>
+__device__ inline void dft8(float2 *v) {
+    // decimation in frequency: a_n = x_n + x_{n+4}, b_n = (x_n - x_{n+4}) W8^n
+    const float r2 = 0.70710678118654752440f;
+    float2 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        a[i] = cadd(v[i], v[i + 4]);
+        b[i] = csub(v[i], v[i + 4]);
+    }
+    // W8^1 = (1 -/+ i)/sqrt2, W8^2 = -/+ i, W8^3 = (-1 -/+ i)/sqrt2
+    {
+        float2 t = b[1];
+        b[1] = INV ? make_float2(r2 * (t.x - t.y), r2 * (t.x + t.y)) : make_float2(r2 * (t.x + t.y), r2 * (t.y - t.x));
+        b[2] = mul_mi<INV>(b[2]);
+        t = b[3];
+        b[3] = INV ? make_float2(-r2 * (t.x + t.y), r2 * (t.x - t.y)) : make_float2(r2 * (t.y - t.x), -r2 * (t.x + t.y));
+    }
+    dft4<INV>(a);
+    dft4<INV>(b);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        v[2 * m] = a[m];
+        v[2 * m + 1] = b[m];
+    }
+}
+
+// Direct odd-radix DFT: y_k = sum_r x_r exp(-/+ 2 pi i r k / R).
+// cos/sin(2 pi m / R) for m < R as literals (folded once the loops are unrolled).
+template <int R>
+__device__ __forceinline__ float odd_cos(int m) {
+    if constexpr (R == 3) {
+        return m == 0 ? 1.0f : -0.5f;
+    } else if constexpr (R == 5) {
+        return m == 0 ? 1.0f : (m == 1 || m == 4) ? 0.30901699437494742410f : -0.80901699437494742410f;
+    } else {
+        return m == 0 ? 1.0f
+               : (m == 1 || m == 6) ? 0.62348980185873353053f
+               : (m == 2 || m == 5) ? -0.22252093395631440429f
+                                    : -0.90096886790241912624f;
+    }
+}
+template <int R>
+__device__ __forceinline__ float odd_sin(int m) {
+    if constexpr (R == 3) {
+        return m == 0 ? 0.0f : (m == 1 ? 0.86602540378443864676f : -0.86602540378443864676f);
+    } else if constexpr (R == 5) {
+        return m == 0 ? 0.0f
+               : m == 1 ? 0.95105651629515357212f
+               : m == 2 ? 0.58778525229247312917f
+               : m == 3 ? -0.58778525229247312917f
+                        : -0.95105651629515357212f;
+    } else {
+        return m == 0 ? 0.0f
+               : m == 1 ? 0.78183148246802980871f
+               : m == 2 ? 0.97492791218182360702f
+               : m == 3 ? 0.43388373911755812048f
+               : m == 4 ? -0.43388373911755812048f
+               : m == 5 ? -0.97492791218182360702f
+                        : -0.78183148246802980871f;
+    }
+}
+
+template <int R, bool INV>
+__device__ inline void dft_odd(float2 *v) {
+    float2 y[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        float2 acc = v[0];
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            const int m = (r * k) % R;
+            // exp(-i 2 pi m/R) = c - i s (forward); c + i s (inverse)
+            const float c = odd_cos<R>(m);
+            const float s = INV ? odd_sin<R>(m) : -odd_sin<R>(m);
+            acc.x += v[r].x * c - v[r].y * s;
+            acc.y += v[r].x * s + v[r].y * c;
+        }
+        y[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = y[k];
+}
+
+template <int R, bool INV>
+__device__ inline void dft(float2 *v) {
+    if constexpr (R == 2) dft2<INV>(v);
+    else if constexpr (R == 4) dft4<INV>(v);
+    else if constexpr (R == 8) dft8<INV>(v);
+    else dft_odd<R, INV>(v);
+}
+
+// One radix-R Stockham stage.  Threads are split into groups of tpt threads per
+// transform (t = tid / tpt); a thread handles butterflies j = jl + b*tpt.
+template <int R, int E, bool INV>
+__device__ void fft_stage(float2 *buf, int stride, int T, int N, int Ns, int lgNs, const float2 *__restrict__ tw,
+                          int t, int jl, int tpt) {
+    constexpr int MB = (E + R - 1) / R;
+    float2 v[MB][R];
+    const int nb = N / R;
+    const int twstep = N / (Ns * R);
+    const bool active = t < T;
+    float2 *row = buf + t * stride;
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        const int j = jl + b * tpt;
+        if (active && j < nb) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = row[j + r * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        const int j = jl + b * tpt;
+        if (active && j < nb) {
+            const int jq = lgNs >= 0 ? (j >> lgNs) : j / Ns;
+            const int k = j - jq * Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    float2 w = tw[k * r * twstep];
+                    if (INV) w.y = -w.y;
+                    v[b][r] = cmul(v[b][r], w);
+                }
+            }
+            dft<R, INV>(v[b]);
+            float2 *dst = row + jq * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) dst[r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Threads per transform for length N (each thread holds at most E elements).
+__host__ __device__ inline int fft_tpt(int N, int E) { return (N + E - 1) / E; }
+
+// Transform T rows of length p.n held in LDS (buf[t*stride + i]).  Needs
+// blockDim.x >= T * fft_tpt(p.n, E).  Caller must __syncthreads() before.
+template <int E, bool INV>
+__device__ void fft_lds(float2 *buf, int stride, int T, const FftPlan &p, const float2 *__restrict__ twall,
+                        int tid, int nthr) {
+    (void)nthr;
+    const float2 *tw = twall + p.tw_off;
+    const int tpt = fft_tpt(p.n, E);
+    const int t = tid / tpt, jl = tid - t * tpt;
+    int Ns = 1;
+    for (int st = 0; st < p.nst; st++) {
+        const int R = p.radix[st];
+        const int lg = (Ns & (Ns - 1)) == 0 ? __builtin_ctz(Ns) : -1;
+        switch (R) {
+            case 8: fft_stage<8, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
+            case 4: fft_stage<4, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
+            case 2: fft_stage<2, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
+            case 3: fft_stage<3, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
+            case 5: fft_stage<5, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
+            case 7: fft_stage<7, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
+            default: break;
+        }
+        Ns *= R;
+    }
+}
+
+}  // namespace rtv

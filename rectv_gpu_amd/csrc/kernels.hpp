@@ -1,0 +1,715 @@
+// HIP kernels of the rectv reconstruction path (gfx950).
+//
+// Data layouts in HBM (z = slice within the current slice batch):
+//   f        [nz][M][n][n]                 f32   time frames of each slice
+//   H        [zb][M][n][2n]                c64   row-transformed spectra (forward),
+//                                                 column-transformed (adjoint)
+//   G        [zb][M][2n][2n]               c64   oversampled spectra, centred
+//   gq       [zb][ntheta][n]               c64   polar samples (gather out / scatter in)
+//   v        [zb][M][n][n]                 c64   adjoint images per basis function
+//   psi, mu  [nchunk][nzp+1][M+1][n+1][n+1] f32x4 dual variables (x, y, t, z)
+// The oversampled grid index kb in [0, 2n) carries the centred frequency
+// kb - n; the checkerboard (-1)^(a+b) before and after the 2D FFT (the
+// reference's fftshift on the (2n+2K)^2 grid, kernels_radonusfft.cuh:26-38) is
+// applied separably in the row and column passes, and the periodic border of
+// that grid (wrap, kernels_radonusfft.cuh:40-64) becomes index arithmetic mod 2n.
+#pragma once
+#include "common.hpp"
+#include "fft_lds.hpp"
+
+namespace rtv {
+
+constexpr int FFT_E = 32;       // complex elements per thread in the LDS FFT
+constexpr int FFT_THREADS = 512;
+constexpr int TS = 16;          // spectral tile edge (cells) for gather/scatter
+constexpr int KW = 9;           // Gaussian window taps per axis, 2K+1 with K = 4
+constexpr int TH = TS + KW - 1; // tile + halo rows loaded by the gather
+constexpr int TP = TH + 1;      // LDS row pitch of the gather tile
+
+__device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
+
+struct SampleGeom {
+    int row0;   // first window row in the 2n grid (already mod 2n)
+    int col0;   // first window column
+    float fy;   // Y - floor(Y), Y = 2n * y0
+    float fx;   // X - floor(X)
+};
+
+struct TileEntry {
+    int tile;
+    int start;
+    int count;
+    int pad;
+};
+
+// ---------------------------------------------------------------------------
+// F1: basis decomposition + deapodisation + zero-padded row FFT.
+// Replaces makecomplex/decphi (kernels_operators.cuh:86-116) and divphi FWD +
+// the first half of the 2D cuFFT (kernels_radonusfft.cuh:4-24, radonusfft.cu:85-89).
+// One workgroup per (slice z, image row y, group of T basis functions).
+__global__ void __launch_bounds__(FFT_THREADS)
+k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float2 *__restrict__ phit,
+           const float *__restrict__ deap, FftPlan plan, const float2 *__restrict__ tw, int n, int M, int T,
+           int stride) {
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int ngr = (M + T - 1) / T;
+    const int jg = L % ngr;
+    const int y = (L / ngr) % n;
+    const int z = L / ngr / n;
+    const int j0 = jg * T;
+    const int Tc = min(T, M - j0);
+    const int N = 2 * n, h = n / 2;
+    const size_t plane = (size_t)n * n;
+    const float *fz = f + (size_t)z * M * plane + (size_t)y * n;
+    const float dy = deap[y];
+    for (int idx = tid; idx < Tc * n; idx += nthr) {
+        const int jt = idx / n, x = idx - jt * n;
+        const float2 *cj = phit + (size_t)(j0 + jt) * M;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int i = 0; i < M; i++) {
+            const float v = fz[(size_t)i * plane + x];
+            const float2 c = cj[i];  // phi_j[t_i]; use its conjugate
+            acc.x += v * c.x;
+            acc.y -= v * c.y;
+        }
+        const float s = deap[x] * dy * sgn(x + h);
+        lds[jt * stride + x + h] = cscale(acc, s);
+        const int q = x < h ? x : x + n;  // zero padding
+        lds[jt * stride + q] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_lds<FFT_E, false>(lds, stride, Tc, plan, tw, tid, nthr);
+    float2 *Hz = H + ((size_t)z * M + j0) * n * N + (size_t)y * N;
+    for (int idx = tid; idx < Tc * N; idx += nthr) {
+        const int jt = idx / N, kx = idx - jt * N;
+        Hz[(size_t)jt * n * N + kx] = cscale(lds[jt * stride + kx], sgn(kx));
+    }
+}
+
+// F2: zero-padded column FFT -> oversampled spectrum G (second half of the 2D
+// cuFFT of radonusfft.cu:89 with the post-checkerboard).  One workgroup per
+// (z, j, tile of T adjacent columns).
+__global__ void __launch_bounds__(FFT_THREADS)
+k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, const float2 *__restrict__ tw,
+           int n, int M, int T, int stride) {
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int N = 2 * n, h = n / 2;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int ntile = N / T;
+    const int kt = L % ntile;
+    const int zj = L / ntile;  // z*M + j
+    const int kx0 = kt * T;
+    const float2 *Hs = H + (size_t)zj * n * N + kx0;
+    for (int idx = tid; idx < T * n; idx += nthr) {
+        const int t = idx % T, y = idx / T;
+        lds[t * stride + y + h] = cscale(Hs[(size_t)y * N + t], sgn(y + h));
+        const int q = y < h ? y : y + n;
+        lds[t * stride + q] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_lds<FFT_E, false>(lds, stride, T, plan, tw, tid, nthr);
+    float2 *Gs = G + (size_t)zj * N * N + kx0;
+    for (int idx = tid; idx < T * N; idx += nthr) {
+        const int t = idx % T, ky = idx / T;
+        Gs[(size_t)ky * N + t] = cscale(lds[t * stride + ky], sgn(ky));
+    }
+}
+
+// Gaussian window weights exp(-beta (a - K - frac)^2), a = 0..8 (the separable
+// factors of kernels_radonusfft.cuh:114-118; the PI/mu factor is folded into
+// the ray tables).
+__device__ __forceinline__ void window_weights(float frac, float beta, float *w) {
+#pragma unroll
+    for (int a = 0; a < KW; a++) {
+        const float d = (float)(a - 4) - frac;
+        w[a] = expf(-beta * d * d);
+    }
+}
+
+// F3: tile-binned Gaussian gather from the M spectra of slice z, contracted with
+// phi_j[k] (replaces gather FWD kernels_radonusfft.cuh:81-144 run once per basis
+// function, plus mulphi/addreal's sum over j, operators.cu:22-31).  One workgroup
+// per (sample tile entry, z); the tile plus its 8-cell halo of JC spectra is
+// staged in LDS; each thread owns samples.  Output gq[z][k][s] (s >= 1).
+__global__ void __launch_bounds__(256)
+k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
+         const int *__restrict__ ids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik,
+         int n, int nt, int M, int JC, float beta, int nentries) {
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int N = 2 * n, nT = N / TS;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int e = L % nentries;
+    const int z = L / nentries;
+    const TileEntry te = tiles[e];
+    const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
+    const float2 *Gz = G + (size_t)z * M * N * N;
+    float2 *out = gq + (size_t)z * nt * n;
+    for (int jc = 0; jc < M; jc += JC) {
+        const int Jc = min(JC, M - jc);
+        if (jc > 0) __syncthreads();
+        for (int idx = tid; idx < Jc * TH * TH; idx += nthr) {
+            const int jl = idx / (TH * TH);
+            const int rc = idx - jl * TH * TH;
+            const int r = rc / TH, c = rc - r * TH;
+            int gy = ty0 + r;
+            if (gy >= N) gy -= N;
+            int gx = tx0 + c;
+            if (gx >= N) gx -= N;
+            lds[(jl * TH + r) * TP + c] = Gz[((size_t)(jc + jl) * N + gy) * N + gx];
+        }
+        __syncthreads();
+        for (int p = tid; p < te.count; p += nthr) {
+            const int id = ids[te.start + p];
+            const SampleGeom g = geo[id];
+            const int ly = g.row0 - ty0, lx = g.col0 - tx0;
+            float wy[KW], wx[KW];
+            window_weights(g.fy, beta, wy);
+            window_weights(g.fx, beta, wx);
+            const int k = id / n;
+            float2 acc = (jc == 0) ? make_float2(0.f, 0.f) : out[id];
+            for (int jl = 0; jl < Jc; jl++) {
+                const float2 *t0 = lds + (jl * TH + ly) * TP + lx;
+                float2 sum = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int a = 0; a < KW; a++) {
+                    float2 rs = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int b = 0; b < KW; b++) {
+                        const float2 gv = t0[a * TP + b];
+                        rs.x += wx[b] * gv.x;
+                        rs.y += wx[b] * gv.y;
+                    }
+                    sum.x += wy[a] * rs.x;
+                    sum.y += wy[a] * rs.y;
+                }
+                acc = cadd(acc, cmul(phik[(size_t)k * M + jc + jl], sum));
+            }
+            out[id] = acc;
+        }
+    }
+}
+
+// F4: per-ray 1D transforms (one workgroup per T rays).
+//   FWD: h1[s'] = Re{post_f[s'] * IFFT(pre_f[s] * gq[s])} (- g)   (shift, fftshift,
+//        CUFFT_INVERSE, fftshift, mulr, mulphi scale: radonusfft.cu:96-102,
+//        kernels_operators.cuh:135-149), optional residual norm partials.
+//   ADJ: q[s] = post_a[s] * FFT(pre_a[s'] * h1[s'])  (radonusfft.cu:115-121 and
+//        the 1/n, s == 0 rule of the adjoint gather, kernels_radonusfft.cuh:102-108).
+enum { RAY_FWD = 1, RAY_SUBG = 2, RAY_ADJ = 4 };
+__global__ void __launch_bounds__(FFT_THREADS)
+k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restrict__ h1in,
+      float *__restrict__ h1out, const float2 *__restrict__ pre_f, const float *__restrict__ post_f,
+      const float *__restrict__ pre_a, const float2 *__restrict__ post_a, FftPlan plan,
+      const float2 *__restrict__ tw, int n, int nrays, int T, int stride, int mode,
+      double *__restrict__ partial) {
+    extern __shared__ float2 lds[];
+    __shared__ double red[FFT_THREADS / 64];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int r0 = blockIdx.x * T;
+    const int Tc = min(T, nrays - r0);
+    double acc = 0.0;
+    if (mode & RAY_FWD) {
+        for (int idx = tid; idx < Tc * n; idx += nthr) {
+            const int t = idx / n, s = idx - t * n;
+            float2 v = make_float2(0.f, 0.f);
+            if (s != 0) v = cmul(gq[(size_t)(r0 + t) * n + s], pre_f[s]);
+            lds[t * stride + s] = v;
+        }
+        __syncthreads();
+        fft_lds<FFT_E, true>(lds, stride, Tc, plan, tw, tid, nthr);
+        for (int idx = tid; idx < Tc * n; idx += nthr) {
+            const int t = idx / n, s = idx - t * n;
+            const size_t o = (size_t)(r0 + t) * n + s;
+            float val = lds[t * stride + s].x * post_f[s];
+            if (mode & RAY_SUBG) val -= g[o];
+            acc += (double)val * val;
+            if (h1out) h1out[o] = val;
+            lds[t * stride + s] = make_float2(val * pre_a[s], 0.f);
+        }
+    } else {
+        for (int idx = tid; idx < Tc * n; idx += nthr) {
+            const int t = idx / n, s = idx - t * n;
+            lds[t * stride + s] = make_float2(h1in[(size_t)(r0 + t) * n + s] * pre_a[s], 0.f);
+        }
+    }
+    if (mode & RAY_ADJ) {
+        __syncthreads();
+        fft_lds<FFT_E, false>(lds, stride, Tc, plan, tw, tid, nthr);
+        for (int idx = tid; idx < Tc * n; idx += nthr) {
+            const int t = idx / n, s = idx - t * n;
+            float2 v = make_float2(0.f, 0.f);
+            if (s != 0) v = cmul(lds[t * stride + s], post_a[s]);
+            gq[(size_t)(r0 + t) * n + s] = v;
+        }
+    }
+    if (partial) {
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < (nthr + 63) / 64; w++) s += red[w];
+            partial[blockIdx.x] = s;
+        }
+    }
+}
+
+// A1: tile-owned Gaussian scatter (replaces gather ADJ with global atomicAdd,
+// kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62, run once per basis
+// function with mulphi(-1), operators.cu:47-56).  Each workgroup owns one
+// TS x TS tile of the 2n grid for all M spectra of slice z, accumulates every
+// sample whose window intersects the tile in LDS, then writes the tile once.
+__global__ void __launch_bounds__(256)
+k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
+          const int *__restrict__ ids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik,
+          int n, int nt, int M, float beta, int ntiles) {
+    extern __shared__ float acc[];  // [M][TS*TS][2]
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int N = 2 * n, nT = N / TS;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int e = L % ntiles;
+    const int z = L / ntiles;
+    const TileEntry te = tiles[e];
+    const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
+    for (int idx = tid; idx < M * TS * TS * 2; idx += nthr) acc[idx] = 0.f;
+    __syncthreads();
+    const float2 *q = gq + (size_t)z * nt * n;
+    for (int p = tid; p < te.count; p += nthr) {
+        const int id = ids[te.start + p];
+        const SampleGeom g = geo[id];
+        int dy = g.row0 - ty0;
+        if (dy < 0) dy += N;
+        const int ly0 = dy < TS ? dy : dy - N;
+        int dx = g.col0 - tx0;
+        if (dx < 0) dx += N;
+        const int lx0 = dx < TS ? dx : dx - N;
+        float wy[KW], wx[KW];
+        window_weights(g.fy, beta, wy);
+        window_weights(g.fx, beta, wx);
+        const int k = id / n;
+        const float2 qv = q[id];
+        const int a0 = max(0, -ly0), a1 = min(KW, TS - ly0);
+        const int b0 = max(0, -lx0), b1 = min(KW, TS - lx0);
+        for (int j = 0; j < M; j++) {
+            const float2 pj = cmulc(qv, phik[(size_t)k * M + j]);  // q * conj(phi_j[k])
+            float *aj = acc + (size_t)j * TS * TS * 2;
+            for (int a = a0; a < a1; a++) {
+                const float wa = wy[a];
+                float *row = aj + ((ly0 + a) * TS + lx0) * 2;
+                for (int b = b0; b < b1; b++) {
+                    const float w = wa * wx[b];
+                    atomicAdd(row + 2 * b, w * pj.x);
+                    atomicAdd(row + 2 * b + 1, w * pj.y);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    float2 *Gz = G + (size_t)z * M * N * N;
+    for (int idx = tid; idx < M * TS * TS; idx += nthr) {
+        const int j = idx / (TS * TS);
+        const int cell = idx - j * TS * TS;
+        const int r = cell / TS, c = cell - r * TS;
+        Gz[((size_t)j * N + ty0 + r) * N + tx0 + c] = make_float2(acc[2 * idx], acc[2 * idx + 1]);
+    }
+}
+
+// A2: column inverse FFT of G keeping the n centre rows (cuFFT INVERSE of
+// radonusfft.cu:128 with the checkerboards, first half).
+__global__ void __launch_bounds__(FFT_THREADS)
+k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, const float2 *__restrict__ tw,
+           int n, int M, int T, int stride) {
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int N = 2 * n, h = n / 2;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int ntile = N / T;
+    const int kt = L % ntile;
+    const int zj = L / ntile;
+    const int kx0 = kt * T;
+    const float2 *Gs = G + (size_t)zj * N * N + kx0;
+    for (int idx = tid; idx < T * N; idx += nthr) {
+        const int t = idx % T, ky = idx / T;
+        lds[t * stride + ky] = cscale(Gs[(size_t)ky * N + t], sgn(ky));
+    }
+    __syncthreads();
+    fft_lds<FFT_E, true>(lds, stride, T, plan, tw, tid, nthr);
+    float2 *Hs = H + (size_t)zj * n * N + kx0;
+    for (int idx = tid; idx < T * n; idx += nthr) {
+        const int t = idx % T, y = idx / T;
+        Hs[(size_t)y * N + t] = cscale(lds[t * stride + y + h], sgn(y + h));
+    }
+}
+
+// A3: row inverse FFT keeping the n centre columns, deapodisation (divphi ADJ,
+// kernels_radonusfft.cuh:19-23).  One workgroup per (z, j, T rows).
+__global__ void __launch_bounds__(FFT_THREADS)
+k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__restrict__ deap, FftPlan plan,
+           const float2 *__restrict__ tw, int n, int M, int T, int stride) {
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int N = 2 * n, h = n / 2;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int nyt = (n + T - 1) / T;
+    const int yt = L % nyt;
+    const int zj = L / nyt;
+    const int y0 = yt * T;
+    const int Tc = min(T, n - y0);
+    const float2 *Hs = H + ((size_t)zj * n + y0) * N;
+    for (int idx = tid; idx < Tc * N; idx += nthr) {
+        const int t = idx / N, kx = idx - t * N;
+        lds[t * stride + kx] = cscale(Hs[(size_t)t * N + kx], sgn(kx));
+    }
+    __syncthreads();
+    fft_lds<FFT_E, true>(lds, stride, Tc, plan, tw, tid, nthr);
+    float2 *vs = v + ((size_t)zj * n + y0) * n;
+    for (int idx = tid; idx < Tc * n; idx += nthr) {
+        const int t = idx / n, x = idx - t * n;
+        const float s = sgn(x + h) * deap[x] * deap[y0 + t];
+        vs[(size_t)t * n + x] = cscale(lds[t * stride + x + h], s);
+    }
+}
+
+// f[z][i] (+)= tau * Re sum_j v_j phi_j[t_i]   (recphi + addreal,
+// kernels_operators.cuh:118-133 and :75-84).  Used by the standalone R^* entry point.
+__global__ void k_recphi(float *__restrict__ f, const float2 *__restrict__ v, const float2 *__restrict__ phit,
+                         int n, int M, float tau, int accumulate) {
+    const size_t plane = (size_t)n * n;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (p >= plane) return;
+    const float2 *vz = v + (size_t)z * M * plane + p;
+    float *fz = f + (size_t)z * M * plane + p;
+    for (int i = 0; i < M; i++) {
+        float s = 0.f;
+        for (int j = 0; j < M; j++) {
+            const float2 a = vz[(size_t)j * plane];
+            const float2 c = phit[(size_t)j * M + i];
+            s += a.x * c.x - a.y * c.y;
+        }
+        const float val = tau * s;
+        fz[(size_t)i * plane] = accumulate ? fz[(size_t)i * plane] + val : val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// TV / ADMM kernels.  Scales (kernels_operators.cuh:40-43, 69-72):
+//   gradient: x, y, z: 1/(cbrt2*lambda1); t: 1/cbrt2
+//   div:      x, y, z: 1/(2 cbrt2 lambda1); t: 1/(2 cbrt2)
+struct TvArgs {
+    int n, M, nzp, nchunk;  // nchunk = total chunks (border flags)
+    int chunk0;             // first chunk handled by this launch
+    float gs, gt, ks, kt;
+    float rho, lam0;
+};
+
+__device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i, int y, int x) {
+    return ((((size_t)c * (a.nzp + 1) + zl) * (a.M + 1) + i) * (a.n + 1) + y) * (a.n + 1) + x;
+}
+
+// U: one inner ADMM step of the TV part fused with the R^* update (admm.cu:85-97):
+//   out = fm - 0.5*div(rho*(grad fm - psi) + mu) - 0.5*Re sum_j v_j phi_j[t_i]
+// Thread per (chunk, y, x) column marching over the chunk's slices and frames.
+// Halo slabs (chunk neighbours) come from A, the previous outer iterate
+// (rectv.cu:170-171; clamped borders kernels_operators.cuh:4-20).
+template <int MM>
+__global__ void __launch_bounds__(256)
+k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__restrict__ out,
+         const float4 *__restrict__ psi, const float4 *__restrict__ mu, const float2 *__restrict__ v,
+         const float2 *__restrict__ phit, TvArgs a, int zv0) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int c = a.chunk0 + blockIdx.z;
+    const int n = a.n, M = a.M, nzp = a.nzp;
+    if (x >= n || y >= n) return;
+    const bool flgl = c != 0, flgr = c != a.nchunk - 1;
+    const size_t plane = (size_t)n * n, S = (size_t)M * plane;
+    const size_t zb = (size_t)c * nzp;
+    const size_t col = (size_t)y * n + x;
+    const int xm = x > 0 ? x - 1 : x, xp = x < n - 1 ? x + 1 : x;
+    const int ym = y > 0 ? y - 1 : y, yp = y < n - 1 ? y + 1 : y;
+    float fz[MM], fzm[MM], fzp[MM], pw[MM], mw[MM];
+#pragma unroll
+    for (int i = 0; i < MM; i++)
+        if (i < M) fz[i] = fm[zb * S + i * plane + col];
+#pragma unroll
+    for (int i = 0; i < MM; i++)
+        if (i < M) fzm[i] = flgl ? A[(zb - 1) * S + i * plane + col] : fz[i];
+#pragma unroll
+    for (int i = 0; i < MM; i++) {
+        if (i < M) {
+            const size_t d = dual_idx(a, c, 0, i + 1, y + 1, x + 1);
+            pw[i] = psi[d].w;
+            mw[i] = mu[d].w;
+        }
+    }
+    for (int zl = 0; zl < nzp; zl++) {
+        const size_t zg = zb + zl;
+#pragma unroll
+        for (int i = 0; i < MM; i++) {
+            if (i < M) {
+                if (zl + 1 < nzp) fzp[i] = fm[(zg + 1) * S + i * plane + col];
+                else fzp[i] = flgr ? A[(zg + 1) * S + i * plane + col] : fz[i];
+            }
+        }
+        float2 vr[MM];
+        if (v) {
+            const float2 *vz = v + ((size_t)(zg - zv0) * M) * plane + col;
+#pragma unroll
+            for (int j = 0; j < MM; j++)
+                if (j < M) vr[j] = vz[(size_t)j * plane];
+        }
+        const size_t d0 = dual_idx(a, c, zl + 1, 0, y + 1, x + 1);
+        float pt = psi[d0].z, mt = mu[d0].z;
+        const float *fl = fm + zg * S;
+#pragma unroll
+        for (int i = 0; i < MM; i++) {
+            if (i < M) {
+                const float f0 = fz[i];
+                const float *fi = fl + i * plane;
+                const float fxp = fi[(size_t)y * n + xp], fxm = fi[(size_t)y * n + xm];
+                const float fyp = fi[(size_t)yp * n + x], fym = fi[(size_t)ym * n + x];
+                // register indices must stay compile-time constants (i is unrolled)
+                const float ftp = (i + 1 < M) ? fz[(i + 1 < MM) ? i + 1 : i] : f0;
+                const float ftm = (i > 0) ? fz[(i > 0) ? i - 1 : 0] : f0;
+                const size_t d = dual_idx(a, c, zl + 1, i + 1, y + 1, x + 1);
+                const float4 ps = psi[d], ms = mu[d];
+                const float px = psi[d - 1].x, mx = mu[d - 1].x;
+                const float py = psi[d - (n + 1)].y, my = mu[d - (n + 1)].y;
+                const float r = a.rho;
+                const float hx0 = r * (a.gs * (fxp - f0) - ps.x) + ms.x;
+                const float hy0 = r * (a.gs * (fyp - f0) - ps.y) + ms.y;
+                const float ht0 = r * (a.gt * (ftp - f0) - ps.z) + ms.z;
+                const float hz0 = r * (a.gs * (fzp[i] - f0) - ps.w) + ms.w;
+                const float hxm = r * (a.gs * (f0 - fxm) - px) + mx;
+                const float hym = r * (a.gs * (f0 - fym) - py) + my;
+                const float htm = r * (a.gt * (f0 - ftm) - pt) + mt;
+                const float hzm = r * (a.gs * (f0 - fzm[i]) - pw[i]) + mw[i];
+                const float dv = a.ks * (hxm - hx0) + a.ks * (hym - hy0) + a.kt * (htm - ht0) + a.ks * (hzm - hz0);
+                float val = f0 - 0.5f * dv;
+                if (v) {
+                    float s = 0.f;
+#pragma unroll
+                    for (int j = 0; j < MM; j++) {
+                        if (j < M) {
+                            const float2 cj = phit[(size_t)j * M + i];
+                            s += vr[j].x * cj.x - vr[j].y * cj.y;
+                        }
+                    }
+                    val -= 0.5f * s;
+                }
+                out[zg * S + i * plane + col] = val;
+                pt = ps.z;
+                mt = ms.z;
+                pw[i] = ps.w;
+                mw[i] = ms.w;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < MM; i++) {
+            if (i < M) {
+                fzm[i] = fz[i];
+                fz[i] = fzp[i];
+            }
+        }
+    }
+}
+
+// P1: after the inner steps: h2 = grad fm; psi = shrink(h2 + mu/rho, lam0/rho);
+// mu += rho (h2 - psi); partial sums r = |h2 - psi|^2, s = |grad(f_old - fm)|^2
+// and ||fm - f_old||^2 (admm.cu:101-117, rectv.cu:224-237).  Thread per dual
+// column (chunk, y', x') over all (n+1)^2 positions, ghost entries included.
+template <int MM>
+__global__ void __launch_bounds__(256)
+k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__restrict__ psi,
+       float4 *__restrict__ mu, TvArgs a, double *__restrict__ partial) {
+    __shared__ double red[3][4];
+    const int xq = blockIdx.x * 64 + (threadIdx.x & 63);  // x' in [0, n]
+    const int yq = blockIdx.y * 4 + (threadIdx.x >> 6);   // y' in [0, n]
+    const int c = a.chunk0 + blockIdx.z;
+    const int n = a.n, M = a.M, nzp = a.nzp;
+    double sr = 0.0, ss = 0.0, sc = 0.0;
+    if (xq <= n && yq <= n) {
+        const bool flgl = c != 0, flgr = c != a.nchunk - 1;
+        const size_t plane = (size_t)n * n, S = (size_t)M * plane;
+        const size_t zb = (size_t)c * nzp;
+        auto cl = [n](int q) { return q < 0 ? 0 : (q > n - 1 ? n - 1 : q); };
+        const int yc = cl(yq - 1), xc = cl(xq - 1), yn = cl(yq), xn = cl(xq);
+        // slab of extension index zl' (E index), returns global slab and whether it is a halo
+        auto slab = [&](int zlq, bool &halo) -> size_t {
+            int q = zlq - 1;
+            const int lo = flgl ? -1 : 0, hi = flgr ? nzp : nzp - 1;
+            q = q < lo ? lo : (q > hi ? hi : q);
+            halo = (q < 0 || q >= nzp);
+            return zb + q;  // q == -1 -> previous chunk's last slab (unsigned wrap avoided: zb >= 1 then)
+        };
+        float cF[MM], cD[MM], nF[MM], nD[MM];
+        {
+            bool hl;
+            const size_t s0 = slab(0, hl);
+#pragma unroll
+            for (int i = 0; i < MM; i++) {
+                if (i < M) {
+                    const size_t o = s0 * S + i * plane + (size_t)yc * n + xc;
+                    const float av = A[o];
+                    cF[i] = hl ? av : fm[o];
+                    cD[i] = hl ? 0.f : av - cF[i];
+                }
+            }
+        }
+        for (int zq = 0; zq <= nzp; zq++) {
+            bool hn, hc;
+            const size_t sn = slab(zq + 1, hn);
+            const size_t sc0 = slab(zq, hc);
+            float xF[MM], xD[MM], yF[MM], yD[MM];
+#pragma unroll
+            for (int i = 0; i < MM; i++) {
+                if (i < M) {
+                    const size_t on = sn * S + i * plane + (size_t)yc * n + xc;
+                    const float an = A[on];
+                    nF[i] = hn ? an : fm[on];
+                    nD[i] = hn ? 0.f : an - nF[i];
+                    const size_t ox = sc0 * S + i * plane + (size_t)yc * n + xn;
+                    const float ax = A[ox];
+                    xF[i] = hc ? ax : fm[ox];
+                    xD[i] = hc ? 0.f : ax - xF[i];
+                    const size_t oy = sc0 * S + i * plane + (size_t)yn * n + xc;
+                    const float ay = A[oy];
+                    yF[i] = hc ? ay : fm[oy];
+                    yD[i] = hc ? 0.f : ay - yF[i];
+                }
+            }
+#pragma unroll
+            for (int iq = 0; iq <= MM; iq++) {
+                if (iq <= M) {
+                    // E index iq reads frame clamp(iq-1); iq+1 reads clamp(iq) (iq <= M)
+                    const int ic = iq > 0 ? iq - 1 : 0;
+                    const int it = iq < MM ? iq : MM - 1;
+                    const float e0 = cF[ic];
+                    const float hx = a.gs * (xF[ic] - e0);
+                    const float hy = a.gs * (yF[ic] - e0);
+                    const float ht = a.gt * ((iq < M ? cF[it] : e0) - e0);
+                    const float hz = a.gs * (nF[ic] - e0);
+                    const float d0 = cD[ic];
+                    const float dx = a.gs * (xD[ic] - d0), dy = a.gs * (yD[ic] - d0);
+                    const float dt = a.gt * ((iq < M ? cD[it] : d0) - d0), dz = a.gs * (nD[ic] - d0);
+                    ss += (double)dx * dx + (double)dy * dy + (double)dt * dt + (double)dz * dz;
+                    if (zq >= 1 && iq >= 1 && yq >= 1 && xq >= 1) sc += (double)d0 * d0;
+                    const size_t d = dual_idx(a, c, zq, iq, yq, xq);
+                    float4 m4 = mu[d];
+                    const float r = a.rho;
+                    float4 p4 = make_float4(hx + m4.x / r, hy + m4.y / r, ht + m4.z / r, hz + m4.w / r);
+                    const float za = sqrtf(p4.x * p4.x + p4.y * p4.y + p4.z * p4.z + p4.w * p4.w);
+                    const float th = a.lam0 / r;
+                    if (za <= th) {
+                        p4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    } else {
+                        p4.x -= th * p4.x / za;
+                        p4.y -= th * p4.y / za;
+                        p4.z -= th * p4.z / za;
+                        p4.w -= th * p4.w / za;
+                    }
+                    m4.x += r * hx - r * p4.x;
+                    m4.y += r * hy - r * p4.y;
+                    m4.z += r * ht - r * p4.z;
+                    m4.w += r * hz - r * p4.w;
+                    psi[d] = p4;
+                    mu[d] = m4;
+                    const float ex = hx - p4.x, ey = hy - p4.y, et = ht - p4.z, ez = hz - p4.w;
+                    sr += (double)ex * ex + (double)ey * ey + (double)et * et + (double)ez * ez;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < MM; i++) {
+                if (i < M) {
+                    cF[i] = nF[i];
+                    cD[i] = nD[i];
+                }
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_down(sr, o, 64);
+        ss += __shfl_down(ss, o, 64);
+        sc += __shfl_down(sc, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = sr;
+        red[1][w] = ss;
+        red[2][w] = sc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        for (int q = 0; q < 3; q++) partial[3 * b + q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
+    }
+}
+
+// Deterministic final reduction of `count` groups of `width` doubles (fixed order).
+__global__ void __launch_bounds__(1024) k_reduce(const double *__restrict__ partial, double *__restrict__ out,
+                                                 int count, int width) {
+    __shared__ double red[1024];
+    for (int q = 0; q < width; q++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < count; b += blockDim.x) s += partial[(size_t)b * width + q];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[q] = red[0];
+        __syncthreads();
+    }
+}
+
+// Standalone gradient of one chunk (extendf + gradf, kernels_operators.cuh:4-44):
+// fext [nzp+2][M][n][n] with halo slabs, h2 AoS float4 [nzp+1][M+1][n+1][n+1].
+__global__ void k_grad(const float *__restrict__ fext, float4 *__restrict__ h2, int n, int M, int nzp, int flgl,
+                       int flgr, float gs, float gt) {
+    const size_t total = (size_t)(nzp + 1) * (M + 1) * (n + 1) * (n + 1);
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    const int xq = p % (n + 1);
+    const int yq = (p / (n + 1)) % (n + 1);
+    const int iq = (p / ((size_t)(n + 1) * (n + 1))) % (M + 1);
+    const int zq = p / ((size_t)(n + 1) * (n + 1) * (M + 1));
+    const size_t plane = (size_t)n * n, S = (size_t)M * plane;
+    auto cl = [](int q, int lo, int hi) { return q < lo ? lo : (q > hi ? hi : q); };
+    auto E = [&](int zz, int ii, int yy, int xx) {
+        const int zs = cl(zz - 1, -flgl, nzp - 1 + flgr) + 1;
+        return fext[zs * S + cl(ii - 1, 0, M - 1) * plane + (size_t)cl(yy - 1, 0, n - 1) * n + cl(xx - 1, 0, n - 1)];
+    };
+    const float e0 = E(zq, iq, yq, xq);
+    h2[p] = make_float4(gs * (E(zq, iq, yq, xq + 1) - e0), gs * (E(zq, iq, yq + 1, xq) - e0),
+                        gt * (E(zq, iq + 1, yq, xq) - e0), gs * (E(zq + 1, iq, yq, xq) - e0));
+}
+
+// Standalone divergence (div, kernels_operators.cuh:46-73): f += tau * div(h2).
+__global__ void k_div(float *__restrict__ f, const float4 *__restrict__ h2, int n, int M, int nzp, float ks,
+                      float kt, float tau) {
+    const size_t total = (size_t)nzp * M * n * n;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    const int x = p % n;
+    const int y = (p / n) % n;
+    const int i = (p / ((size_t)n * n)) % M;
+    const int z = p / ((size_t)n * n * M);
+    auto H = [&](int zz, int ii, int yy, int xx) {
+        return h2[(((size_t)zz * (M + 1) + ii) * (n + 1) + yy) * (n + 1) + xx];
+    };
+    const float4 c = H(z + 1, i + 1, y + 1, x + 1);
+    float val = f[p];
+    val += tau * (H(z + 1, i + 1, y + 1, x).x - c.x) * ks;
+    val += tau * (H(z + 1, i + 1, y, x + 1).y - c.y) * ks;
+    val += tau * (H(z + 1, i, y + 1, x + 1).z - c.z) * kt;
+    val += tau * (H(z, i + 1, y + 1, x + 1).w - c.w) * ks;
+    f[p] = val;
+}
+
+}  // namespace rtv

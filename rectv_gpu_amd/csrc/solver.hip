@@ -1,0 +1,903 @@
+// Host side of the rectv HIP library: FP64 geometry and tables, device memory
+// plan, the ADMM driver and the extern "C" entry points declared in
+// include/rectv_amd.h.
+#include "kernels.hpp"
+#include "../../include/rectv_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace rtv {
+
+static thread_local std::string g_last_error;
+
+static const double PI_REF = 3.1415926535;  // kernels_radonusfft.cuh:1
+static const double PI_EXACT = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- FFT plans
+static bool make_plan(int N, FftPlan &p) {
+    p.n = N;
+    p.nst = 0;
+    int r = N;
+    const int order[] = {8, 4, 2, 3, 5, 7};
+    for (int R : order) {
+        while (r % R == 0 && p.nst < FFT_MAX_STAGES) {
+            p.radix[p.nst++] = R;
+            r /= R;
+        }
+    }
+    return r == 1;
+}
+
+// ---------------------------------------------------------------- geometry
+struct HostGeom {
+    std::vector<SampleGeom> geo;
+    std::vector<int> fids, aids;
+    std::vector<TileEntry> ftiles, atiles;
+    std::vector<float2> pre_f, post_a, phik, phit;
+    std::vector<float> post_f, pre_a, deap;
+    float beta = 0.f;
+};
+
+// USFFT constants, radonusfft.cu:10-12: mu = -log(eps)/(2 n^2) held in float.
+static float usfft_mu(int n) {
+    const float eps = 1e-3f;
+    return (float)(-std::log((double)eps) / (2.0 * (double)n * (double)n));
+}
+static int usfft_K(int n, float mu) {
+    const double m = mu, eps = (double)1e-3f;
+    return (int)std::ceil(2.0 * n * 1 / PI_REF * std::sqrt(-m * std::log(eps) + (m * n) * (m * n) / 4));
+}
+
+static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *theta, const float *phi, float center,
+                           int chunk_max) {
+    const int N = 2 * n, nT = N / TS, K = 4;
+    const float mu = usfft_mu(n);
+    if (usfft_K(n, mu) != K) throw Error(-1, "USFFT half-width differs from 4 for this n");
+    const double mud = mu;
+    hg.beta = (float)(PI_REF * PI_REF / (mud * 4.0 * n * n));
+    const size_t ns = (size_t)nt * n;
+    hg.geo.assign(ns, SampleGeom{0, 0, 0.f, 0.f});
+    std::vector<int> ftile(ns, -1);
+    // polar sample positions (takexy, kernels_radonusfft.cuh:66-79) in FP64
+    for (int k = 0; k < nt; k++) {
+        const double th = (double)theta[k];
+        const double sn = std::sin(th), cs = std::cos(th);
+        for (int s = 1; s < n; s++) {
+            const double r = (double)(s - n / 2) / n;
+            double x0 = r * sn, y0 = r * cs;
+            if (x0 >= 0.5) x0 = 0.5 - 1e-5;
+            if (y0 >= 0.5) y0 = 0.5 - 1e-5;
+            const double X = 2.0 * n * x0, Y = 2.0 * n * y0;
+            const double fxl = std::floor(X), fyl = std::floor(Y);
+            SampleGeom g;
+            g.col0 = (int)(((long long)n + (long long)fxl - K) % N + N) % N;
+            g.row0 = (int)(((long long)n + (long long)fyl - K) % N + N) % N;
+            g.fx = (float)(X - fxl);
+            g.fy = (float)(Y - fyl);
+            const size_t id = (size_t)k * n + s;
+            hg.geo[id] = g;
+            ftile[id] = (g.row0 / TS) * nT + g.col0 / TS;
+        }
+    }
+    // forward lists: each sample in the tile holding its window origin
+    const int ntile = nT * nT;
+    {
+        std::vector<int> cnt(ntile, 0), off(ntile + 1, 0);
+        for (size_t id = 0; id < ns; id++)
+            if (ftile[id] >= 0) cnt[ftile[id]]++;
+        for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
+        hg.fids.assign(off[ntile], 0);
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (size_t id = 0; id < ns; id++)
+            if (ftile[id] >= 0) hg.fids[pos[ftile[id]]++] = (int)id;
+        hg.ftiles.clear();
+        for (int t = 0; t < ntile; t++)
+            for (int s0 = 0; s0 < cnt[t]; s0 += chunk_max)
+                hg.ftiles.push_back(TileEntry{t, off[t] + s0, std::min(chunk_max, cnt[t] - s0), 0});
+        std::stable_sort(hg.ftiles.begin(), hg.ftiles.end(),
+                         [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
+    }
+    // adjoint lists: each sample in every tile its 9x9 window intersects
+    {
+        std::vector<int> cnt(ntile, 0);
+        auto tiles_of = [&](const SampleGeom &g, int *out) {
+            const int ty[2] = {g.row0 / TS, ((g.row0 + KW - 1) % N) / TS};
+            const int tx[2] = {g.col0 / TS, ((g.col0 + KW - 1) % N) / TS};
+            int c = 0;
+            for (int a = 0; a < (ty[1] != ty[0] ? 2 : 1); a++)
+                for (int b = 0; b < (tx[1] != tx[0] ? 2 : 1); b++) out[c++] = ty[a] * nT + tx[b];
+            return c;
+        };
+        int tl[4];
+        for (size_t id = 0; id < ns; id++) {
+            if (ftile[id] < 0) continue;
+            const int c = tiles_of(hg.geo[id], tl);
+            for (int q = 0; q < c; q++) cnt[tl[q]]++;
+        }
+        std::vector<int> off(ntile + 1, 0);
+        for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
+        hg.aids.assign(off[ntile], 0);
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (size_t id = 0; id < ns; id++) {
+            if (ftile[id] < 0) continue;
+            const int c = tiles_of(hg.geo[id], tl);
+            for (int q = 0; q < c; q++) hg.aids[pos[tl[q]]++] = (int)id;
+        }
+        hg.atiles.resize(ntile);
+        for (int t = 0; t < ntile; t++) hg.atiles[t] = TileEntry{t, off[t], cnt[t], 0};
+        std::stable_sort(hg.atiles.begin(), hg.atiles.end(),
+                         [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
+    }
+    // per-ray tables (takeshift kernels_radonusfft.cuh:191-198 with c = -/+(center - n/2),
+    // fftshift signs, 1/n, PI/mu, 1/sqrt(n ntheta), ntheta/sqrt(M))
+    const float cfl = (float)(center - n / 2.0);
+    const double cfw = -(double)cfl, cad = (double)cfl;
+    const double cst = (PI_REF / mud) / n / std::sqrt((double)n * nt) * nt / std::sqrt((double)M);
+    hg.pre_f.resize(n);
+    hg.post_f.resize(n);
+    hg.pre_a.resize(n);
+    hg.post_a.resize(n);
+    for (int s = 0; s < n; s++) {
+        const double sg = 1 - 2 * ((s + 1) % 2);
+        const double af = 2 * PI_REF * cfw * (s - n / 2.0) / n;
+        const double aa = 2 * PI_REF * cad * (s - n / 2.0) / n;
+        hg.pre_f[s] = s == 0 ? make_float2(0.f, 0.f) : make_float2((float)(sg * std::cos(af)), (float)(sg * std::sin(af)));
+        hg.post_f[s] = (float)(sg * cst);
+        hg.pre_a[s] = (float)sg;
+        hg.post_a[s] = s == 0 ? make_float2(0.f, 0.f)
+                              : make_float2((float)(sg * cst * std::cos(aa)), (float)(sg * cst * std::sin(aa)));
+    }
+    // deapodisation 1/(2n exp(-mu (x - n/2)^2)) per axis (divphi, kernels_radonusfft.cuh:11-22)
+    hg.deap.resize(n);
+    for (int x = 0; x < n; x++) {
+        const double t = x - n / 2;
+        hg.deap[x] = (float)(1.0 / (2.0 * n * std::exp(-mud * t * t)));
+    }
+    // basis tables: phik[k][j] = phi_j[k], phit[j][i] = phi_j[t_i], t_i = i*nt/M (int)
+    hg.phik.resize((size_t)nt * M);
+    for (int j = 0; j < M; j++)
+        for (int k = 0; k < nt; k++)
+            hg.phik[(size_t)k * M + j] = make_float2(phi[(size_t)j * 2 * nt + 2 * k], phi[(size_t)j * 2 * nt + 2 * k + 1]);
+    hg.phit.resize((size_t)M * M);
+    for (int j = 0; j < M; j++)
+        for (int i = 0; i < M; i++) hg.phit[(size_t)j * M + i] = hg.phik[(size_t)(i * nt / M) * M + j];
+}
+
+// ---------------------------------------------------------------- device buffers
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count, size_t &total) {
+        release(total);
+        if (count == 0) return;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(-3, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+        }
+        n = count;
+        total += count * sizeof(T);
+    }
+    void release(size_t &total) {
+        if (p) {
+            (void)hipFree(p);
+            total -= n * sizeof(T);
+        }
+        p = nullptr;
+        n = 0;
+    }
+};
+
+template <typename T>
+static void upload(DBuf<T> &b, const std::vector<T> &v, size_t &total, hipStream_t st) {
+    b.alloc(std::max<size_t>(v.size(), 1), total);
+    if (!v.empty()) RTV_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+struct Solver {
+    int n, nt, M, nz, nzp, nch, ngpus, N;
+    int dev = 0;
+    hipStream_t st = nullptr;
+    size_t bytes = 0;
+    bool freed = false;
+    int lds_max = 65536;
+    // FFT
+    FftPlan plan_n{}, plan_N{};
+    DBuf<float2> d_tw;
+    int T_rows = 1, T_cols = 1, T_ray = 1, T_arows = 1, st_n = 0, st_N = 0, JC = 1;
+    // state
+    DBuf<float> d_f[3];
+    int ia = 0;
+    DBuf<float4> d_psi, d_mu;
+    DBuf<float> d_g;
+    float rho = 0.5f;
+    // scratch (one batch of cb chunks = zb slices)
+    int cb = 1, zb = 1;
+    DBuf<float2> d_H, d_G, d_v, d_gq;
+    DBuf<float> d_h1;
+    DBuf<double> d_part, d_red;
+    // geometry
+    bool inputs = false;
+    DBuf<SampleGeom> d_geo;
+    DBuf<int> d_fids, d_aids;
+    DBuf<TileEntry> d_ftiles, d_atiles;
+    DBuf<float2> d_phik, d_phit, d_pre_f, d_post_a;
+    DBuf<float> d_post_f, d_pre_a, d_deap;
+    int nf_entries = 0, na_tiles = 0;
+    float beta = 0.f;
+    // profiling
+    bool prof = false;
+    struct Rec {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> evpool;
+    std::map<std::string, std::pair<long long, double>> stats;
+
+    Solver(int n_, int nt_, int m_, int nz_, int nzp_, int ngpus_) : n(n_), nt(nt_), M(m_), nz(nz_), nzp(nzp_) {
+        RTV_REQUIRE(n > 0 && nt > 0 && M > 0 && nz > 0 && nzp > 0, "all sizes must be positive");
+        RTV_REQUIRE(nz % nzp == 0, "nz must be a multiple of nzp (the reference silently drops the remainder)");
+        RTV_REQUIRE(n % 8 == 0, "n must be a multiple of 8 (16-cell spectral tiles of the 2n grid)");
+        RTV_REQUIRE(M <= 32, "at most 32 basis functions are supported");
+        nch = nz / nzp;
+        ngpus = std::max(1, std::min(ngpus_, nch));
+        try {
+            init_device();
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+
+    void init_device() {
+        N = 2 * n;
+        RTV_REQUIRE(make_plan(n, plan_n) && make_plan(N, plan_N),
+                    "n must factor into radices 2, 3, 5, 7 (FFT sizes n and 2n)");
+        int ndev = 0;
+        RTV_HIP(hipGetDeviceCount(&ndev));
+        RTV_REQUIRE(ndev > 0, "no HIP device available");
+        RTV_HIP(hipGetDevice(&dev));
+        int lm = 0;
+        if (hipDeviceGetAttribute(&lm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lm > 0)
+            lds_max = lm;
+        if (const char *e = std::getenv("RECTV_LDS_MAX")) lds_max = std::min(lds_max, std::atoi(e));
+        lds_max -= 1024;  // room for the kernels' static shared arrays
+        RTV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        setup_fft();
+        // persistent state
+        const size_t V = (size_t)nz * M * n * n;
+        for (auto &b : d_f) b.alloc(V, bytes);
+        const size_t V4 = (size_t)nch * (nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1);
+        d_psi.alloc(V4, bytes);
+        d_mu.alloc(V4, bytes);
+        d_g.alloc((size_t)nz * nt * n, bytes);
+        // scratch plan
+        const size_t per_slice = (size_t)M * n * N * 8 + (size_t)M * N * N * 8 + (size_t)M * n * n * 8 +
+                                 (size_t)nt * n * 8 + (size_t)nt * n * 4;
+        size_t freeb = 0, totb = 0;
+        RTV_HIP(hipMemGetInfo(&freeb, &totb));
+        double budget = 0.45 * (double)freeb;
+        if (const char *e = std::getenv("RECTV_SCRATCH_MB")) budget = std::atof(e) * 1048576.0;
+        cb = (int)std::max(1.0, std::min((double)nch, std::floor(budget / ((double)per_slice * nzp))));
+        zb = cb * nzp;
+        d_H.alloc((size_t)zb * M * n * N, bytes);
+        d_G.alloc((size_t)zb * M * N * N, bytes);
+        d_v.alloc((size_t)zb * M * n * n, bytes);
+        d_gq.alloc((size_t)zb * nt * n, bytes);
+        d_h1.alloc((size_t)zb * nt * n, bytes);
+        const size_t pb = (size_t)((n + 1 + 63) / 64) * ((n + 1 + 3) / 4) * nch;
+        const size_t rb = (size_t)((zb * nt + T_ray - 1) / T_ray);
+        d_part.alloc(3 * std::max(pb, rb) + 16, bytes);
+        d_red.alloc(8, bytes);
+        RTV_HIP(hipStreamSynchronize(st));
+    }
+
+    int fft_threads(int T, int len) const {
+        int t = T * fft_tpt(len, FFT_E);
+        t = ((t + 63) / 64) * 64;
+        return std::max(64, t);
+    }
+
+    void setup_fft() {
+        // twiddle tables exp(-2 pi i m / L) in FP64 -> float for L = n and 2n
+        std::vector<float2> tw;
+        auto add = [&](FftPlan &p) {
+            p.tw_off = (int)tw.size();
+            for (int m = 0; m < p.n; m++) {
+                const double a = -2.0 * PI_EXACT * (double)m / p.n;
+                tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+            }
+        };
+        add(plan_n);
+        add(plan_N);
+        upload(d_tw, tw, bytes, st);
+        st_n = n + 1;
+        st_N = N + 1;
+        // transforms per workgroup: bounded by LDS and by FFT_THREADS threads
+        auto tmax = [&](int len, int stride) {
+            return std::max(1, std::min(FFT_THREADS / fft_tpt(len, FFT_E), lds_max / (8 * stride)));
+        };
+        RTV_REQUIRE(fft_tpt(N, FFT_E) <= FFT_THREADS, "2n-point transform needs too many threads");
+        const int tN = tmax(N, st_N), tn = tmax(n, st_n);
+        RTV_REQUIRE(tN >= 1 && 8 * st_N <= lds_max, "2n-point transform does not fit in LDS");
+        T_rows = std::min(M, tN);
+        T_cols = 1;
+        while (T_cols * 2 <= std::min(tN, 16) && N % (T_cols * 2) == 0) T_cols *= 2;
+        T_arows = std::min(tN, 16);
+        T_ray = std::min(tn, 16);
+        JC = std::max(1, std::min(M, lds_max / (TH * TP * 8)));
+        const int np = (M + JC - 1) / JC;
+        JC = (M + np - 1) / np;
+        const int big = lds_max;
+        RTV_HIP(hipFuncSetAttribute((const void *)k_fwd_rows, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        RTV_HIP(hipFuncSetAttribute((const void *)k_fwd_cols, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        RTV_HIP(hipFuncSetAttribute((const void *)k_adj_cols, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        RTV_HIP(hipFuncSetAttribute((const void *)k_adj_rows, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        RTV_HIP(hipFuncSetAttribute((const void *)k_ray, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        RTV_HIP(hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        RTV_HIP(hipFuncSetAttribute((const void *)k_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+    }
+
+    // ------------------------------------------------------------ profiling
+    hipEvent_t ev() {
+        if (evpool.empty()) {
+            hipEvent_t e;
+            RTV_HIP(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = evpool.back();
+        evpool.pop_back();
+        return e;
+    }
+    template <typename F>
+    void launch(const char *name, F &&f) {
+        if (!prof) {
+            f();
+        } else {
+            hipEvent_t a = ev(), b = ev();
+            RTV_HIP(hipEventRecord(a, st));
+            f();
+            RTV_HIP(hipEventRecord(b, st));
+            pending.push_back(Rec{name, a, b});
+        }
+        RTV_HIP(hipGetLastError());
+    }
+    void collect() {
+        if (pending.empty()) return;
+        RTV_HIP(hipStreamSynchronize(st));
+        for (auto &r : pending) {
+            float ms = 0.f;
+            RTV_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            auto &s = stats[r.name];
+            s.first += 1;
+            s.second += ms;
+            evpool.push_back(r.a);
+            evpool.push_back(r.b);
+        }
+        pending.clear();
+    }
+
+    // ------------------------------------------------------------ inputs
+    void set_inputs(const float *g, const float *theta, const float *phi, float center) {
+        RTV_REQUIRE(g && theta && phi, "null input pointer");
+        for (size_t i = 0; i < (size_t)M * 2 * nt; i++)
+            RTV_REQUIRE(std::isfinite(phi[i]), "phi contains non-finite values");
+        for (int k = 0; k < nt; k++) RTV_REQUIRE(std::isfinite(theta[k]), "theta contains non-finite values");
+        RTV_REQUIRE(std::isfinite(center), "center must be finite");
+        HostGeom hg;
+        build_geometry(hg, n, nt, M, theta, phi, center, 1024);
+        beta = hg.beta;
+        upload(d_geo, hg.geo, bytes, st);
+        upload(d_fids, hg.fids, bytes, st);
+        upload(d_ftiles, hg.ftiles, bytes, st);
+        upload(d_aids, hg.aids, bytes, st);
+        upload(d_atiles, hg.atiles, bytes, st);
+        upload(d_phik, hg.phik, bytes, st);
+        upload(d_phit, hg.phit, bytes, st);
+        upload(d_pre_f, hg.pre_f, bytes, st);
+        upload(d_post_f, hg.post_f, bytes, st);
+        upload(d_pre_a, hg.pre_a, bytes, st);
+        upload(d_post_a, hg.post_a, bytes, st);
+        upload(d_deap, hg.deap, bytes, st);
+        nf_entries = (int)hg.ftiles.size();
+        na_tiles = (int)hg.atiles.size();
+        RTV_HIP(hipMemcpyAsync(d_g.p, g, (size_t)nz * nt * n * sizeof(float), hipMemcpyHostToDevice, st));
+        RTV_HIP(hipStreamSynchronize(st));
+        inputs = true;
+    }
+
+    void reset() {
+        const size_t V = (size_t)nz * M * n * n;
+        for (auto &b : d_f) RTV_HIP(hipMemsetAsync(b.p, 0, V * sizeof(float), st));
+        RTV_HIP(hipMemsetAsync(d_psi.p, 0, d_psi.n * sizeof(float4), st));
+        RTV_HIP(hipMemsetAsync(d_mu.p, 0, d_mu.n * sizeof(float4), st));
+        ia = 0;
+        rho = 0.5f;
+        RTV_HIP(hipStreamSynchronize(st));
+    }
+
+    // ------------------------------------------------------------ operator pipeline
+    // forward: f slices [z0, z0+zc) -> polar samples in d_gq (and h1 / residual / q per mode)
+    void forward_to_rays(const float *f, int zc, const float *gdat, float *h1out, int mode, double *partial) {
+        const size_t S = (size_t)M * n * n;
+        (void)S;
+        {
+            const int ngr = (M + T_rows - 1) / T_rows;
+            const int blocks = zc * n * ngr;
+            const int thr = fft_threads(T_rows, N);
+            const size_t lds = (size_t)T_rows * st_N * sizeof(float2);
+            launch("fwd_rows", [&] {
+                hipLaunchKernelGGL(k_fwd_rows, dim3(blocks), dim3(thr), lds, st, f, d_H.p, d_phit.p, d_deap.p, plan_N,
+                                   d_tw.p, n, M, T_rows, st_N);
+            });
+        }
+        {
+            const int blocks = zc * M * (N / T_cols);
+            const int thr = fft_threads(T_cols, N);
+            const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
+            launch("fwd_cols", [&] {
+                hipLaunchKernelGGL(k_fwd_cols, dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p, n, M,
+                                   T_cols, st_N);
+            });
+        }
+        {
+            const int blocks = nf_entries * zc;
+            const size_t lds = (size_t)JC * TH * TP * sizeof(float2);
+            if (blocks > 0)
+                launch("gather", [&] {
+                    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), lds, st, d_G.p, d_gq.p, d_geo.p, d_fids.p,
+                                       d_ftiles.p, d_phik.p, n, nt, M, JC, beta, nf_entries);
+                });
+        }
+        ray(zc, gdat, nullptr, h1out, mode, partial);
+    }
+
+    void ray(int zc, const float *gdat, const float *h1in, float *h1out, int mode, double *partial) {
+        const int nrays = zc * nt;
+        const int blocks = (nrays + T_ray - 1) / T_ray;
+        const int thr = fft_threads(T_ray, n);
+        const size_t lds = (size_t)T_ray * st_n * sizeof(float2);
+        launch("ray", [&] {
+            hipLaunchKernelGGL(k_ray, dim3(blocks), dim3(thr), lds, st, d_gq.p, gdat, h1in, h1out, d_pre_f.p,
+                               d_post_f.p, d_pre_a.p, d_post_a.p, plan_n, d_tw.p, n, nrays, T_ray, st_n, mode, partial);
+        });
+    }
+
+    // adjoint: d_gq (q samples of zc slices) -> d_v images
+    void rays_to_images(int zc) {
+        {
+            const int blocks = na_tiles * zc;
+            const size_t lds = (size_t)M * TS * TS * 2 * sizeof(float);
+            launch("scatter", [&] {
+                hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), lds, st, d_G.p, d_gq.p, d_geo.p, d_aids.p,
+                                   d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
+            });
+        }
+        {
+            const int blocks = zc * M * (N / T_cols);
+            const int thr = fft_threads(T_cols, N);
+            const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
+            launch("adj_cols", [&] {
+                hipLaunchKernelGGL(k_adj_cols, dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p, n, M,
+                                   T_cols, st_N);
+            });
+        }
+        {
+            const int nyt = (n + T_arows - 1) / T_arows;
+            const int blocks = zc * M * nyt;
+            const int thr = fft_threads(T_arows, N);
+            const size_t lds = (size_t)T_arows * st_N * sizeof(float2);
+            launch("adj_rows", [&] {
+                hipLaunchKernelGGL(k_adj_rows, dim3(blocks), dim3(thr), lds, st, d_H.p, d_v.p, d_deap.p, plan_N,
+                                   d_tw.p, n, M, T_arows, st_N);
+            });
+        }
+    }
+
+    TvArgs tv_args(float lam1, float lam0) const {
+        TvArgs a;
+        a.n = n;
+        a.M = M;
+        a.nzp = nzp;
+        a.nchunk = nch;
+        a.chunk0 = 0;
+        const double c2 = std::cbrt(2.0);
+        a.gs = (float)(1.0 / (c2 * lam1));
+        a.gt = (float)(1.0 / c2);
+        a.ks = (float)(1.0 / (2.0 * c2 * lam1));
+        a.kt = (float)(1.0 / (2.0 * c2));
+        a.rho = rho;
+        a.lam0 = lam0;
+        return a;
+    }
+
+    template <int MM>
+    void launch_update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
+        dim3 grid((n + 63) / 64, (n + 3) / 4, ncb);
+        launch("update", [&] {
+            hipLaunchKernelGGL(k_update<MM>, grid, dim3(256), 0, st, cur, A, nxt, d_psi.p, d_mu.p, d_v.p, d_phit.p,
+                               ta, zv0);
+        });
+    }
+    void update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
+        if (M <= 4) launch_update<4>(cur, A, nxt, ta, ncb, zv0);
+        else if (M <= 8) launch_update<8>(cur, A, nxt, ta, ncb, zv0);
+        else if (M <= 16) launch_update<16>(cur, A, nxt, ta, ncb, zv0);
+        else launch_update<32>(cur, A, nxt, ta, ncb, zv0);
+    }
+    template <int MM>
+    void launch_prox(const float *cur, const float *A, const TvArgs &ta) {
+        dim3 grid((n + 1 + 63) / 64, (n + 1 + 3) / 4, nch);
+        launch("prox", [&] {
+            hipLaunchKernelGGL(k_prox<MM>, grid, dim3(256), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
+        });
+        const int cnt = grid.x * grid.y * grid.z;
+        launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, 3); });
+    }
+    void prox(const float *cur, const float *A, const TvArgs &ta) {
+        if (M <= 4) launch_prox<4>(cur, A, ta);
+        else if (M <= 8) launch_prox<8>(cur, A, ta);
+        else if (M <= 16) launch_prox<16>(cur, A, ta);
+        else launch_prox<32>(cur, A, ta);
+    }
+
+    // ------------------------------------------------------------ ADMM driver
+    void iterate(int niter, int titer, float lam0, float lam1, double *hist) {
+        RTV_REQUIRE(inputs, "inputs not set (call set_inputs / run first)");
+        RTV_REQUIRE(niter >= 0 && titer >= 0, "niter and titer must be >= 0");
+        RTV_REQUIRE(std::isfinite(lam0) && std::isfinite(lam1) && lam1 != 0.f, "lambda1 must be finite and nonzero");
+        const size_t S = (size_t)M * n * n;
+        const bool quiet = std::getenv("RECTV_QUIET") && std::atoi(std::getenv("RECTV_QUIET")) != 0;
+        double red[3];
+        for (int it = 0; it < niter; it++) {
+            TvArgs ta = tv_args(lam1, lam0);
+            const int w1 = (ia + 1) % 3, w2 = (ia + 2) % 3;
+            float *A = d_f[ia].p;
+            int cur = ia;
+            for (int k = 0; k < titer; k++) {
+                const int nxt = (k == 0) ? w1 : (cur == w1 ? w2 : w1);
+                for (int c0 = 0; c0 < nch; c0 += cb) {
+                    const int ncb = std::min(cb, nch - c0);
+                    const int z0 = c0 * nzp, zc = ncb * nzp;
+                    forward_to_rays(d_f[cur].p + z0 * S, zc, d_g.p + (size_t)z0 * nt * n, nullptr,
+                                    RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);
+                    rays_to_images(zc);
+                    TvArgs tb = ta;
+                    tb.chunk0 = c0;
+                    update(d_f[cur].p, A, d_f[nxt].p, tb, ncb, z0);
+                }
+                cur = nxt;
+            }
+            prox(d_f[cur].p, A, ta);
+            RTV_HIP(hipMemcpyAsync(red, d_red.p, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+            RTV_HIP(hipStreamSynchronize(st));
+            collect();
+            const double r = red[0], s = (double)rho * rho * red[1], rdiff = red[2];
+            if (hist) {
+                hist[4 * it + 0] = r;
+                hist[4 * it + 1] = s;
+                hist[4 * it + 2] = rho;
+                hist[4 * it + 3] = rdiff;
+            }
+            // rho rule, rectv.cu:224-232
+            if (r > 10 * s) rho *= 2;
+            else if (s > 10 * r) rho *= 0.5f;
+            if (!quiet) fprintf(stderr, "iter (%d/%d), rdiff ||f(k+1)-f(k)||:%f \n", it, niter, rdiff);
+            ia = cur;
+        }
+    }
+
+    void get_result(float *fres) {
+        RTV_REQUIRE(fres, "null output pointer");
+        RTV_HIP(hipMemcpyAsync(fres, d_f[ia].p, (size_t)nz * M * n * n * sizeof(float), hipMemcpyDeviceToHost, st));
+        RTV_HIP(hipStreamSynchronize(st));
+    }
+
+    // ------------------------------------------------------------ operator entry points
+    void apply_radon(const float *f, float *g) {
+        RTV_REQUIRE(inputs, "inputs not set");
+        const size_t S = (size_t)M * n * n;
+        DBuf<float> tf, th;
+        size_t tmp = 0;
+        tf.alloc((size_t)nz * S, tmp);
+        th.alloc((size_t)nz * nt * n, tmp);
+        RTV_HIP(hipMemcpyAsync(tf.p, f, (size_t)nz * S * sizeof(float), hipMemcpyHostToDevice, st));
+        for (int z0 = 0; z0 < nz; z0 += zb) {
+            const int zc = std::min(zb, nz - z0);
+            forward_to_rays(tf.p + z0 * S, zc, nullptr, th.p + (size_t)z0 * nt * n, RAY_FWD, nullptr);
+        }
+        RTV_HIP(hipMemcpyAsync(g, th.p, (size_t)nz * nt * n * sizeof(float), hipMemcpyDeviceToHost, st));
+        RTV_HIP(hipStreamSynchronize(st));
+        collect();
+        tf.release(tmp);
+        th.release(tmp);
+    }
+
+    void apply_radon_adj_dev(const float *gdev, float *fdev, float tau, int accumulate) {
+        const size_t S = (size_t)M * n * n;
+        for (int z0 = 0; z0 < nz; z0 += zb) {
+            const int zc = std::min(zb, nz - z0);
+            ray(zc, nullptr, gdev + (size_t)z0 * nt * n, nullptr, RAY_ADJ, nullptr);
+            rays_to_images(zc);
+            dim3 grid((unsigned)(((size_t)n * n + 255) / 256), zc);
+            launch("recphi", [&] {
+                hipLaunchKernelGGL(k_recphi, grid, dim3(256), 0, st, fdev + z0 * S, d_v.p, d_phit.p, n, M, tau,
+                                   accumulate);
+            });
+        }
+    }
+
+    void apply_radon_adj(const float *g, float *f) {
+        RTV_REQUIRE(inputs, "inputs not set");
+        const size_t S = (size_t)M * n * n;
+        DBuf<float> tf, th;
+        size_t tmp = 0;
+        tf.alloc((size_t)nz * S, tmp);
+        th.alloc((size_t)nz * nt * n, tmp);
+        RTV_HIP(hipMemcpyAsync(th.p, g, (size_t)nz * nt * n * sizeof(float), hipMemcpyHostToDevice, st));
+        apply_radon_adj_dev(th.p, tf.p, 1.0f, 0);
+        RTV_HIP(hipMemcpyAsync(f, tf.p, (size_t)nz * S * sizeof(float), hipMemcpyDeviceToHost, st));
+        RTV_HIP(hipStreamSynchronize(st));
+        collect();
+        tf.release(tmp);
+        th.release(tmp);
+    }
+
+    void apply_gradient(const float *fext, int iz, float lam1, float *h2) {
+        RTV_REQUIRE(iz >= 0 && iz < nch, "chunk index out of range");
+        const size_t S = (size_t)M * n * n;
+        const size_t D = (size_t)(nzp + 1) * (M + 1) * (n + 1) * (n + 1);
+        DBuf<float> tf;
+        DBuf<float4> th;
+        size_t tmp = 0;
+        tf.alloc((size_t)(nzp + 2) * S, tmp);
+        th.alloc(D, tmp);
+        RTV_HIP(hipMemcpyAsync(tf.p, fext, (size_t)(nzp + 2) * S * sizeof(float), hipMemcpyHostToDevice, st));
+        const TvArgs ta = tv_args(lam1, 0.f);
+        launch("grad", [&] {
+            hipLaunchKernelGGL(k_grad, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, st, tf.p, th.p, n, M, nzp,
+                               iz != 0 ? 1 : 0, iz != nch - 1 ? 1 : 0, ta.gs, ta.gt);
+        });
+        RTV_HIP(hipMemcpyAsync(h2, th.p, D * sizeof(float4), hipMemcpyDeviceToHost, st));
+        RTV_HIP(hipStreamSynchronize(st));
+        collect();
+        tf.release(tmp);
+        th.release(tmp);
+    }
+
+    void apply_divergence(const float *h2, float lam1, float tau, float *f) {
+        const size_t S = (size_t)M * n * n;
+        const size_t D = (size_t)(nzp + 1) * (M + 1) * (n + 1) * (n + 1);
+        DBuf<float> tf;
+        DBuf<float4> th;
+        size_t tmp = 0;
+        tf.alloc((size_t)nzp * S, tmp);
+        th.alloc(D, tmp);
+        RTV_HIP(hipMemcpyAsync(tf.p, f, (size_t)nzp * S * sizeof(float), hipMemcpyHostToDevice, st));
+        RTV_HIP(hipMemcpyAsync(th.p, h2, D * sizeof(float4), hipMemcpyHostToDevice, st));
+        const TvArgs ta = tv_args(lam1, 0.f);
+        const size_t tot = (size_t)nzp * S;
+        launch("div", [&] {
+            hipLaunchKernelGGL(k_div, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, tf.p, th.p, n, M, nzp,
+                               ta.ks, ta.kt, tau);
+        });
+        RTV_HIP(hipMemcpyAsync(f, tf.p, (size_t)nzp * S * sizeof(float), hipMemcpyDeviceToHost, st));
+        RTV_HIP(hipStreamSynchronize(st));
+        collect();
+        tf.release(tmp);
+        th.release(tmp);
+    }
+
+    // rectv::check_adjoints, rectv.cu:255-289
+    void check_adjoints(double *res, const float *g, const float *theta, const float *phi, float lam1, float center) {
+        RTV_REQUIRE(nz == nzp, "check_adjoints requires nz == nzp (one chunk), as in the reference");
+        set_inputs(g, theta, phi, center);
+        const size_t S = (size_t)M * n * n;
+        const size_t V = (size_t)nz * S, Gs = (size_t)nz * nt * n;
+        std::vector<float> f(V), h1(Gs), fn(V);
+        apply_radon_adj(g, f.data());
+        apply_radon(f.data(), h1.data());
+        for (int q = 0; q < 6; q++) res[q] = 0.0;
+        for (size_t k = 0; k < V; k++) res[0] += (double)(f[k] * f[k]);
+        for (size_t k = 0; k < Gs; k++) res[1] += (double)(g[k] * h1[k]);
+        for (size_t k = 0; k < Gs; k++) res[2] += (double)(h1[k] * h1[k]);
+        std::vector<float> fext((size_t)(nzp + 2) * S, 0.f);
+        std::memcpy(fext.data() + S, f.data(), V * sizeof(float));
+        const size_t D = (size_t)(nzp + 1) * (M + 1) * (n + 1) * (n + 1);
+        std::vector<float> h2(4 * D);
+        apply_gradient(fext.data(), 0, lam1, h2.data());
+        std::fill(fn.begin(), fn.end(), 0.f);
+        apply_divergence(h2.data(), lam1, 1.0f, fn.data());
+        const size_t q4 = D;  // first (n+1)^2 (m+1)(nz+1) floats of the float4 array
+        for (size_t k = 0; k < V; k++) res[3] += (double)(f[k] * fn[k]);
+        for (size_t k = 0; k < q4; k++) res[4] += (double)(h2[k] * h2[k]);
+        for (size_t k = 0; k < V; k++) res[5] += (double)(fn[k] * fn[k]);
+    }
+
+    void release() {
+        if (freed) return;
+        (void)hipStreamSynchronize(st);
+        for (auto &r : pending) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        pending.clear();
+        for (auto e : evpool) (void)hipEventDestroy(e);
+        evpool.clear();
+        size_t &t = bytes;
+        d_tw.release(t);
+        for (auto &b : d_f) b.release(t);
+        d_psi.release(t);
+        d_mu.release(t);
+        d_g.release(t);
+        d_H.release(t);
+        d_G.release(t);
+        d_v.release(t);
+        d_gq.release(t);
+        d_h1.release(t);
+        d_part.release(t);
+        d_red.release(t);
+        d_geo.release(t);
+        d_fids.release(t);
+        d_aids.release(t);
+        d_ftiles.release(t);
+        d_atiles.release(t);
+        d_phik.release(t);
+        d_phit.release(t);
+        d_pre_f.release(t);
+        d_post_a.release(t);
+        d_post_f.release(t);
+        d_pre_a.release(t);
+        d_deap.release(t);
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
+        freed = true;
+    }
+    ~Solver() { release(); }
+};
+
+}  // namespace rtv
+
+// ====================================================================== C ABI
+struct rtv_solver {
+    rtv::Solver *s;
+};
+
+template <typename F>
+static int guarded(F &&f) {
+    try {
+        f();
+        rtv::g_last_error.clear();
+        return RTV_OK;
+    } catch (const rtv::Error &e) {
+        rtv::g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        rtv::g_last_error = e.what();
+        return RTV_ERR_HIP;
+    }
+}
+
+static rtv::Solver *live(rtv_solver *h) {
+    if (!h || !h->s || h->s->freed) throw rtv::Error(RTV_ERR_STATE, "solver handle is null or freed");
+    return h->s;
+}
+
+extern "C" {
+
+const char *rtv_last_error(void) { return rtv::g_last_error.c_str(); }
+
+int rtv_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int rtv_create(int n, int ntheta, int m, int nz, int nzp, int ngpus, rtv_solver **out) {
+    return guarded([&] {
+        RTV_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        rtv::Solver *s = new rtv::Solver(n, ntheta, m, nz, nzp, ngpus);
+        *out = new rtv_solver{s};
+    });
+}
+
+int rtv_free(rtv_solver *h) {
+    return guarded([&] {
+        if (!h) return;
+        if (h->s) {
+            h->s->release();
+            delete h->s;
+        }
+        delete h;
+    });
+}
+
+int rtv_set_inputs(rtv_solver *h, const float *g, const float *theta, const float *phi, float center) {
+    return guarded([&] { live(h)->set_inputs(g, theta, phi, center); });
+}
+int rtv_reset(rtv_solver *h) {
+    return guarded([&] { live(h)->reset(); });
+}
+int rtv_iterate(rtv_solver *h, int niter, int titer, float lambda0, float lambda1, double *hist) {
+    return guarded([&] { live(h)->iterate(niter, titer, lambda0, lambda1, hist); });
+}
+int rtv_get_result(rtv_solver *h, float *fres) {
+    return guarded([&] { live(h)->get_result(fres); });
+}
+int rtv_synchronize(rtv_solver *h) {
+    return guarded([&] { RTV_HIP(hipStreamSynchronize(live(h)->st)); });
+}
+
+int rtv_run(rtv_solver *h, float *fres, const float *g, const float *theta, const float *phi, float center,
+            float lambda0, float lambda1, int niter, int titer) {
+    return guarded([&] {
+        rtv::Solver *s = live(h);
+        s->set_inputs(g, theta, phi, center);
+        s->reset();
+        s->iterate(niter, titer, lambda0, lambda1, nullptr);
+        s->get_result(fres);
+    });
+}
+
+int rtv_check_adjoints(rtv_solver *h, double *res, const float *g, const float *theta, const float *phi,
+                       float lambda1, float center) {
+    return guarded([&] {
+        RTV_REQUIRE(res, "null result pointer");
+        live(h)->check_adjoints(res, g, theta, phi, lambda1, center);
+    });
+}
+
+int rtv_apply_radon(rtv_solver *h, const float *f, float *g) {
+    return guarded([&] { live(h)->apply_radon(f, g); });
+}
+int rtv_apply_radon_adj(rtv_solver *h, const float *g, float *f) {
+    return guarded([&] { live(h)->apply_radon_adj(g, f); });
+}
+int rtv_apply_gradient(rtv_solver *h, const float *fext, int iz, float lambda1, float *h2) {
+    return guarded([&] { live(h)->apply_gradient(fext, iz, lambda1, h2); });
+}
+int rtv_apply_divergence(rtv_solver *h, const float *h2, float lambda1, float tau, float *f) {
+    return guarded([&] { live(h)->apply_divergence(h2, lambda1, tau, f); });
+}
+
+int rtv_set_profiling(rtv_solver *h, int on) {
+    return guarded([&] {
+        rtv::Solver *s = live(h);
+        s->collect();
+        s->prof = on != 0;
+        if (on) s->stats.clear();
+    });
+}
+
+int rtv_get_profile(rtv_solver *h, char *buf, int len) {
+    return guarded([&] {
+        rtv::Solver *s = live(h);
+        s->collect();
+        std::string out;
+        char line[256];
+        for (auto &kv : s->stats) {
+            snprintf(line, sizeof line, "%s %lld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+            out += line;
+        }
+        RTV_REQUIRE(buf && len > 0, "null profile buffer");
+        std::strncpy(buf, out.c_str(), (size_t)len - 1);
+        buf[len - 1] = 0;
+    });
+}
+
+long long rtv_device_bytes(rtv_solver *h) {
+    if (!h || !h->s) return -1;
+    return (long long)h->s->bytes;
+}
+
+}  // extern "C"

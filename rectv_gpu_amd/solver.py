@@ -1,0 +1,147 @@
+"""Python drop-in for rectv_gpu.Solver (src/rectv_gpu/solver.py:11-93).
+
+Same constructor, ``recon``/``adjoint_tests`` signatures, defaults and return
+arrays as the reference; inputs are validated (dtype float32, C order, shapes)
+and converted where that is lossless, where the reference would read whatever
+bytes the pointer gives.  Extra methods (``set_inputs``, ``iterate``, ...) expose
+the staged form used for timing.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .rectv import rectv
+
+
+def getp(a):
+    """Host address of a numpy array (src/rectv_gpu/solver.py:7-8)."""
+    return a.__array_interface__['data'][0]
+
+
+def _f32(a, shape, name):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != tuple(shape):
+        if a.size == int(np.prod(shape)):
+            a = a.reshape(shape)
+        else:
+            raise ValueError("%s has shape %s, expected %s" % (name, a.shape, tuple(shape)))
+    return a
+
+
+class Solver(rectv):
+    """Time-resolved tomography solver (ADMM with 4D total variation).
+
+    Parameters are those of the reference: n (detector width / slice size),
+    ntheta (total number of projections), m (number of temporal basis
+    functions), nz (slices), nzp (slices per chunk), ngpus (accepted; this build
+    drives one GPU).
+    """
+
+    def __init__(self, n, ntheta, m, nz, nzp, ngpus):
+        super().__init__(n, ntheta, m, nz, nzp, ngpus)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, type, value, traceback):
+        self.free()
+
+    def _inputs(self, data, theta, phi):
+        data = _f32(data, (self.nz, self.ntheta, self.n), "data")
+        theta = _f32(theta, (self.ntheta,), "theta")
+        phi = _f32(phi, (self.m, 2 * self.ntheta), "phi")
+        return data, theta, phi
+
+    def recon(self, data, theta, phi, rot_center=None, lambda0=1e-3, lambda1=1, niter=1, titer=1):
+        """Reconstruct (src/rectv_gpu/solver.py:45-84).
+
+        data [nz, ntheta, n], theta [ntheta], phi [m, 2*ntheta] (re/im
+        interleaved); returns float32 [nz, m, n, n].  As in the reference the
+        TV weight passed to the native solver is lambda0*lambda1 and the
+        rotation centre defaults to n/2.
+        """
+        lambda0 = lambda0 * lambda1
+        if rot_center is None:
+            rot_center = self.n / 2
+        data, theta, phi = self._inputs(data, theta, phi)
+        rtv = np.zeros([self.nz, self.m, self.n, self.n], dtype='float32')
+        self.run(getp(rtv), getp(data), getp(theta), getp(phi), rot_center, lambda0, lambda1, niter, titer)
+        return rtv
+
+    def adjoint_tests(self, data, theta, phi, lambda1, rot_center=None):
+        """The six inner products of rectv::check_adjoints (src/rectv_gpu/solver.py:86-93)."""
+        if rot_center is None:
+            rot_center = self.n / 2
+        data, theta, phi = self._inputs(data, theta, phi)
+        res = np.zeros([6])
+        self.check_adjoints(getp(res), getp(data), getp(theta), getp(phi), lambda1, rot_center)
+        return res
+
+    # ---- staged API (inputs resident on the device) ---------------------------
+    def set_inputs(self, data, theta, phi, rot_center=None):
+        if rot_center is None:
+            rot_center = self.n / 2
+        data, theta, phi = self._inputs(data, theta, phi)
+        _lib.check(_lib.lib().rtv_set_inputs(self._handle(), getp(data), getp(theta), getp(phi),
+                                             float(rot_center)))
+
+    def reset(self):
+        _lib.check(_lib.lib().rtv_reset(self._handle()))
+
+    def iterate(self, niter, titer, lambda0, lambda1, history=False):
+        """Run niter outer iterations on the resident state (lambda0 used as given).
+        With history=True returns an array [niter, 4] of (r, s, rho, ||f(k+1)-f(k)||^2)."""
+        hist = np.zeros((max(niter, 1), 4)) if history else None
+        _lib.check(_lib.lib().rtv_iterate(self._handle(), int(niter), int(titer), float(lambda0),
+                                          float(lambda1), getp(hist) if history else None))
+        return hist[:niter] if history else None
+
+    def result(self):
+        out = np.zeros([self.nz, self.m, self.n, self.n], dtype='float32')
+        _lib.check(_lib.lib().rtv_get_result(self._handle(), getp(out)))
+        return out
+
+    def synchronize(self):
+        _lib.check(_lib.lib().rtv_synchronize(self._handle()))
+
+    # ---- operator entry points (parity tests) ----------------------------------
+    def apply_radon(self, f):
+        f = _f32(f, (self.nz, self.m, self.n, self.n), "f")
+        g = np.zeros((self.nz, self.ntheta, self.n), np.float32)
+        _lib.check(_lib.lib().rtv_apply_radon(self._handle(), getp(f), getp(g)))
+        return g
+
+    def apply_radon_adj(self, g):
+        g = _f32(g, (self.nz, self.ntheta, self.n), "g")
+        f = np.zeros((self.nz, self.m, self.n, self.n), np.float32)
+        _lib.check(_lib.lib().rtv_apply_radon_adj(self._handle(), getp(g), getp(f)))
+        return f
+
+    def apply_gradient(self, fext, iz, lambda1):
+        fext = _f32(fext, (self.nzp + 2, self.m, self.n, self.n), "fext")
+        h2 = np.zeros((self.nzp + 1, self.m + 1, self.n + 1, self.n + 1, 4), np.float32)
+        _lib.check(_lib.lib().rtv_apply_gradient(self._handle(), getp(fext), int(iz), float(lambda1), getp(h2)))
+        return h2
+
+    def apply_divergence(self, h2, lambda1, tau, f):
+        h2 = _f32(h2, (self.nzp + 1, self.m + 1, self.n + 1, self.n + 1, 4), "h2")
+        f = _f32(f, (self.nzp, self.m, self.n, self.n), "f").copy()
+        _lib.check(_lib.lib().rtv_apply_divergence(self._handle(), getp(h2), float(lambda1), float(tau), getp(f)))
+        return f
+
+    def set_profiling(self, on=True):
+        _lib.check(_lib.lib().rtv_set_profiling(self._handle(), 1 if on else 0))
+
+    def profile(self):
+        """{kernel: (launches, total_ms)} from HIP events on the solver's stream."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        _lib.check(_lib.lib().rtv_get_profile(self._handle(), buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
+
+    def device_bytes(self):
+        return int(_lib.lib().rtv_device_bytes(self._handle()))

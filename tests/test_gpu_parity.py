@@ -1,0 +1,225 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances (FP32 arithmetic on the GPU vs FP64 oracle, relative L2):
+  * each operator (R_apr, R_apr^*, gradient, divergence): 1e-5
+  * check_adjoints values: 1e-5
+  * full ADMM reconstruction after niter outer iterations: 1e-4, with the same
+    rho sequence.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import rectv_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+OP_TOL = 1e-5
+RUN_TOL = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def _case(name):
+    from rectv_gpu_amd import basis as B
+    rng = np.random.default_rng(abs(hash(name)) % 2 ** 32)
+    if name == "n64_rand":
+        n, nt, m, nz = 64, 48, 4, 2
+        th = np.sort(rng.uniform(0, 8 * np.pi, nt)).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2 + 1.25
+    elif name == "n72_mixed_radix_m3":
+        n, nt, m, nz = 72, 30, 3, 1
+        th = B.angles(15, 2)
+        phi = rng.standard_normal((m, 2 * nt)).astype(np.float32) / np.sqrt(nt)
+        center = n / 2 - 0.5
+    elif name == "cfg_small":
+        n, nt, m, nz = 128, 128, 4, 8
+        th = B.angles(32, 4)
+        phi, center = B.realbasis(nt, m), n / 2
+    else:
+        raise KeyError(name)
+    return n, nt, m, nz, th, phi, center, rng
+
+
+CASES = ["n64_rand", "n72_mixed_radix_m3", "cfg_small"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_radon_forward_parity(gpu, name):
+    from rectv_gpu_amd import Solver
+    n, nt, m, nz, th, phi, center, rng = _case(name)
+    say("case", name, "forward R_apr", n, nt, m, nz)
+    f = rng.standard_normal((nz, m, n, n)).astype(np.float32)
+    g0 = np.zeros((nz, nt, n), np.float32)
+    with Solver(n, nt, m, nz, nz, 1) as s:
+        s.set_inputs(g0, th, phi, center)
+        out = s.apply_radon(f)
+    ref = O.Operators(n, nt, m, th, phi, center, 1.0).radonapr(f.astype(np.float64))
+    e = rel(out, ref)
+    say("  rel L2", e)
+    assert e < OP_TOL
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_radon_adjoint_parity(gpu, name):
+    from rectv_gpu_amd import Solver
+    n, nt, m, nz, th, phi, center, rng = _case(name)
+    say("case", name, "adjoint R_apr^*", n, nt, m, nz)
+    g = rng.standard_normal((nz, nt, n)).astype(np.float32)
+    with Solver(n, nt, m, nz, nz, 1) as s:
+        s.set_inputs(g, th, phi, center)
+        out = s.apply_radon_adj(g)
+    ref = O.Operators(n, nt, m, th, phi, center, 1.0).radonapradj(np.zeros((nz, m, n, n)), g.astype(np.float64))
+    e = rel(out, ref)
+    say("  rel L2", e)
+    assert e < OP_TOL
+
+
+def test_gpu_operators_are_adjoint(gpu):
+    from rectv_gpu_amd import Solver
+    n, nt, m, nz, th, phi, center, rng = _case("n64_rand")
+    say("case adjointness of the GPU operators")
+    f = rng.standard_normal((nz, m, n, n)).astype(np.float32)
+    g = rng.standard_normal((nz, nt, n)).astype(np.float32)
+    with Solver(n, nt, m, nz, nz, 1) as s:
+        s.set_inputs(g, th, phi, center)
+        Rf = s.apply_radon(f)
+        Rsg = s.apply_radon_adj(g)
+    a = np.vdot(g.astype(np.float64), Rf)
+    b = np.vdot(Rsg.astype(np.float64), f)
+    say("  <Rf,g>", a, "<f,R*g>", b)
+    assert abs(a - b) / abs(a) < 1e-5
+
+
+@pytest.mark.parametrize("iz", [0, 1, 2])
+def test_gradient_divergence_parity(gpu, iz):
+    from rectv_gpu_amd import Solver, takephi
+    n, nt, m, nzp, nz, lam = 24, 16, 3, 3, 9, 2.5
+    say("case gradient/divergence chunk", iz)
+    rng = np.random.default_rng(iz)
+    fext = rng.standard_normal((nzp + 2, m, n, n)).astype(np.float32)
+    flgl, flgr = int(iz != 0), int(iz != nz // nzp - 1)
+    with Solver(n, nt, m, nz, nzp, 1) as s:
+        s.set_inputs(np.zeros((nz, nt, n), np.float32), np.zeros(nt, np.float32), takephi(nt, m))
+        h2 = s.apply_gradient(fext, iz, lam)
+        f0 = rng.standard_normal((nzp, m, n, n)).astype(np.float32)
+        fd = s.apply_divergence(h2, lam, -0.5, f0)
+    ref = O.gradf(O.extendf(fext.astype(np.float64), flgl, flgr), lam)
+    e1 = rel(h2, ref)
+    refd = O.div(f0.astype(np.float64), ref, -0.5, lam)
+    e2 = rel(fd - f0, refd - f0)
+    say("  grad", e1, "div", e2)
+    assert e1 < OP_TOL and e2 < OP_TOL
+
+
+def test_check_adjoints_parity(gpu):
+    from rectv_gpu_amd import Solver, takephi
+    n, nt, m, nz = 64, 64, 4, 1
+    say("case check_adjoints", n, nt, m, nz)
+    phi = takephi(nt, m)
+    np.random.seed(0)
+    data = np.random.random([nz, nt, n]).astype('float32')
+    theta = np.linspace(0, 8 * np.pi, nt).astype('float32')
+    with Solver(n, nt, m, nz, nz, 1) as s:
+        res = s.adjoint_tests(data, theta, phi, 8.0)
+    ref = O.check_adjoints(data, theta, phi, n, nt, m, nz, nz, 8.0, n / 2)
+    say("  gpu", res, "\n  ref", ref)
+    assert np.all(np.abs(res - ref) <= OP_TOL * np.abs(ref))
+
+
+def test_reference_adjoint_test_full_size(gpu):
+    """tests/test_adjoint.py:14-57 of the reference at its own size."""
+    from rectv_gpu_amd import Solver, takephi
+    n, ntheta, m, nz, nzp, ngpus, lambda1 = 504, 2400, 16, 1, 1, 1, 8
+    say("case reference adjoint test n=504 ntheta=2400 m=16")
+    phi = takephi(ntheta, m)
+    np.random.seed(0)
+    data = np.random.random([nz, ntheta, n]).astype('float32')
+    theta = np.linspace(0, 8 * np.pi, ntheta).astype('float32')
+    with Solver(n, ntheta, m, nz, nzp, ngpus) as slv:
+        res = slv.adjoint_tests(data, theta, phi, lambda1)
+    say("  res", res)
+    np.testing.assert_allclose(res[0], res[1], rtol=1e-1)
+    np.testing.assert_allclose(res[1], res[2], rtol=1)
+    np.testing.assert_allclose(res[3], res[4], rtol=1e-1)
+    np.testing.assert_allclose(res[4], res[5], rtol=1)
+    # a true adjoint pair: far tighter than the reference's rtol
+    assert abs(res[0] - res[1]) < 1e-4 * res[0]
+
+
+def _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed, basis="fourier", lam0=1e-3, lam1=4.0):
+    from rectv_gpu_amd import Solver, synthetic as S, basis as B
+    th = B.angles(ntr, nrot)
+    nt = len(th)
+    phi = B.takephi(nt, m) if basis == "fourier" else B.realbasis(nt, m)
+    sph = S.spheres(n, nz, 6, seed, nrot=nrot)
+    data = S.project(sph, n, nz, th, ntr, m, noise=0.01, seed=seed)
+    with Solver(n, nt, m, nz, nzp, 1) as s:
+        s.set_inputs(data, th, phi)
+        s.reset()
+        hist = s.iterate(niter, titer, lam0 * lam1, lam1, history=True)
+        out = s.result()
+    h = []
+    ref = O.run(data, th, phi, n, nt, m, nz, nzp, n / 2, lam0 * lam1, lam1, niter, titer, history=h)
+    return out, ref, hist, h
+
+
+@pytest.mark.parametrize("n,nz,nzp,nrot,ntr,m,niter,titer", [
+    (32, 6, 2, 2, 16, 4, 5, 2),      # three chunks: halo slabs from the previous iterate
+    (64, 4, 4, 2, 32, 4, 4, 3),
+    (48, 2, 1, 4, 12, 3, 3, 1),      # nzp = 1, odd m
+])
+def test_run_parity_small(gpu, n, nz, nzp, nrot, ntr, m, niter, titer):
+    say("case run", n, nz, nzp, nrot * ntr, m, niter, titer)
+    out, ref, hist, h = _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed=n + nz)
+    e = rel(out, ref)
+    say("  rel L2", e, "rho gpu", list(hist[:, 2]), "rho ref", [x["rho"] for x in h])
+    assert e < RUN_TOL
+    assert list(hist[:, 2]) == [x["rho"] for x in h]
+    np.testing.assert_allclose(hist[:, 3], [x["rdiff"] for x in h], rtol=1e-3)
+
+
+def test_run_parity_small_config(gpu):
+    """The plain-method small configuration: N=128, Nz=8, Nrot=4 x 32, M=4 real basis,
+    20 spheres, 1% noise, lambda0=1e-3, lambda1=4, niter=32, titer=4."""
+    from rectv_gpu_amd import Solver, synthetic as S
+    w = S.workload("small")
+    say("case run small config", w["n"], w["nz"], w["ntheta"], w["m"], w["niter"])
+    titer = 4
+    with Solver(w["n"], w["ntheta"], w["m"], w["nz"], w["nzp"], 1) as s:
+        out = s.recon(w["data"], w["theta"], w["phi"], lambda0=w["lambda0"], lambda1=w["lambda1"],
+                      niter=w["niter"], titer=titer)
+    h = []
+    ref = O.run(w["data"], w["theta"], w["phi"], w["n"], w["ntheta"], w["m"], w["nz"], w["nzp"], w["center"],
+                w["lambda0"] * w["lambda1"], w["lambda1"], w["niter"], titer, history=h)
+    e = rel(out, ref)
+    say("  rel L2", e)
+    assert e < RUN_TOL
+
+
+def test_run_edge_cases(gpu):
+    from rectv_gpu_amd import Solver, takephi
+    n, nt, m, nz = 32, 16, 2, 2
+    say("case niter=0 / titer=0 / staged == run")
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    phi = takephi(nt, m)
+    data = np.random.default_rng(1).random((nz, nt, n)).astype(np.float32)
+    with Solver(n, nt, m, nz, 1, 1) as s:
+        assert not s.recon(data, th, phi, niter=0).any()
+        z = s.recon(data, th, phi, niter=2, titer=0)
+        assert np.isfinite(z).all()
+        a = s.recon(data, th, phi, lambda0=1e-3, lambda1=2.0, niter=3, titer=2)
+        s.set_inputs(data, th, phi)
+        s.reset()
+        s.iterate(1, 2, 2e-3, 2.0)
+        s.iterate(2, 2, 2e-3, 2.0)
+        b = s.result()
+    assert rel(b, a) < 1e-6
