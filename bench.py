@@ -1,0 +1,170 @@
+"""Benchmark: ADMM outer iterations of the 4D reconstruction on one MI355X.
+
+One step = one outer ADMM iteration (titer inner steps + prox/dual update + rho
+rule) over the whole workload, with every input already resident in HBM.
+Default workload: N=1024, Nz=64, Nzp=16, Nrot=8 x 200 = 1600 projections, M=16,
+titer=4 (synthetic moving-sphere data).  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config full|mid|small] [--titer 4] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "voxel-iter/s at N=1024 M=16 (% HBM roofline); batched/served vs plain; p50/p99"
+HBM_PEAK = 8.0e12
+
+
+def kernel_bytes(name, n, nt, m, zc, nzp):
+    """Algorithmic HBM bytes of one launch of each kernel over zc slices (each input
+    read once, each output written once)."""
+    N = 2 * n
+    V = zc * m * n * n                       # voxels
+    V4 = (zc // nzp) * (nzp + 1) * (m + 1) * (n + 1) ** 2  # dual entries
+    S = zc * m                               # spectra
+    Gs = zc * nt * n                         # polar samples
+    return {
+        "fwd_rows": 4 * V + S * n * N * 8,
+        "fwd_cols": S * (n * N + N * N) * 8,
+        "gather": S * N * N * 8 + Gs * 8,
+        "ray": Gs * (8 + 4 + 8),
+        "scatter": Gs * 8 + S * N * N * 8,
+        "adj_cols": S * (N * N + n * N) * 8,
+        "adj_rows": S * (n * N + n * n) * 8,
+        "update": V * (4 + 4 + 8) + 32 * V4,
+        "prox": 8 * V + 48 * V4,
+    }.get(name)
+
+
+def byte_model(n, nt, m, nz, nzp, titer):
+    """Compulsory HBM bytes of one outer iteration (fused-pipeline model)."""
+    V = n * n * m * nz
+    V4 = (n + 1) ** 2 * (m + 1) * (nzp + 1) * nz // nzp
+    S = m * nz
+    G = (2 * n) ** 2 * 8
+    H = 2 * n * n * 8
+    Gs = n * nt * nz
+    inner = (4 * V + S * (2 * H + 2 * G) + 8 * Gs) + (4 * Gs + S * (2 * G + 2 * H) + 8 * V) + 32 * V4
+    return titer * inner + 12 * V + 48 * V4
+
+
+def cpu_baseline(w, titer):
+    """FP32 NumPy port of the same solver (the oracle in float32) on a bounded sample:
+    one slice of the workload (all projections, all basis functions), one inner step
+    and one prox/dual update, single thread.  Scaled to voxel-iter/s of a titer-step
+    outer iteration."""
+    from oracle import rectv_oracle as O
+    O.set_workers(1)
+    n, nt, m = w["n"], w["ntheta"], w["m"]
+    dt = np.float32
+    ops = O.Operators(n, nt, m, w["theta"], w["phi"], w["center"], w["lambda1"], dt)
+    g = w["data"][:1].astype(dt)
+    fext = np.zeros((3, m, n, n), dt)
+    fext[1] = np.random.default_rng(0).random((m, n, n)).astype(dt) * 0.1
+    psi = np.zeros((2, m + 1, n + 1, n + 1, 4), dt)
+    mu = np.zeros_like(psi)
+    t0 = time.perf_counter()
+    O.solver_admm(ops, fext, g, psi, mu, w["lambda0"] * w["lambda1"], 0.5, 0, 0, 1)
+    t1 = time.perf_counter()
+    O.solver_admm(ops, fext, g, psi, mu, w["lambda0"] * w["lambda1"], 0.5, 0, 0, 0)
+    t2 = time.perf_counter()
+    t_prox = t2 - t1
+    t_inner = max(t1 - t0 - t_prox, 1e-9)
+    t_iter = titer * t_inner + t_prox
+    return {"value": n * n * m / t_iter, "unit": "voxel-iter/s", "cores": 1, "kind": "port",
+            "sample": "1 slice (n=%d, %d projections, M=%d): one inner ADMM step %.2f s + prox %.2f s, "
+                      "FP32 NumPy/SciPy, 1 thread; value = n^2*M / (titer*inner + prox) with titer=%d"
+                      % (n, nt, m, t_inner, t_prox, titer)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--titer", type=int, default=4)
+    ap.add_argument("--config", default="full")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build drives exactly one GPU; --gpus %d is not supported "
+              "(the z-chunk sharding across GPUs is described in DESIGN.md, not built)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("RECTV_QUIET", "1")
+    from rectv_gpu_amd import Solver, synthetic
+
+    t_setup = time.perf_counter()
+    w = synthetic.workload(a.config, seed=a.seed)
+    n, nt, m, nz, nzp = w["n"], w["ntheta"], w["m"], w["nz"], w["nzp"]
+    lam0, lam1 = w["lambda0"] * w["lambda1"], w["lambda1"]
+    print("bench: data generated (%.1f s); config %s n=%d nz=%d nzp=%d ntheta=%d m=%d titer=%d"
+          % (time.perf_counter() - t_setup, a.config, n, nz, nzp, nt, m, a.titer), file=sys.stderr, flush=True)
+    s = Solver(n, nt, m, nz, nzp, 1)
+    s.set_inputs(w["data"], w["theta"], w["phi"], w["center"])
+    s.reset()
+    print("bench: solver ready, %.1f GB on device" % (s.device_bytes() / 1e9), file=sys.stderr, flush=True)
+    if a.warmup > 0:
+        s.iterate(a.warmup, a.titer, lam0, lam1)
+    s.synchronize()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    # timed region; per-kernel HIP events are recorded on the solver's stream
+    s.set_profiling(True)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.iterate(a.steps, a.titer, lam0, lam1)
+    s.synchronize()
+    t1 = time.perf_counter()
+    prof = s.profile()
+    s.set_profiling(False)
+    ms = (t1 - t0) * 1e3 / a.steps
+    vox = n * n * nz * m
+    value = vox * a.steps / (t1 - t0)
+    print("bench: %.1f ms per outer iteration" % ms, file=sys.stderr, flush=True)
+    zb = nz  # kernels run over the whole slice batch
+    kern = {}
+    for name, (cnt, tot) in prof.items():
+        b = kernel_bytes(name, n, nt, m, zb, nzp)
+        avg = tot / cnt
+        kern[name] = {"launches": cnt, "avg_ms": round(avg, 4), "share": 0.0}
+        if b is not None:
+            kern[name]["alg_bytes"] = b
+            kern[name]["gbps"] = round(b / (avg * 1e-3) / 1e9, 1)
+    total = sum(t for _, t in prof.values())
+    for name, (cnt, tot) in prof.items():
+        kern[name]["share"] = round(tot / total, 4)
+    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    db = kernel_bytes(dom, n, nt, m, zb, nzp)
+    davg = prof[dom][1] / prof[dom][0] * 1e-3
+    achieved = db / davg / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4), "traffic": None}
+    B = byte_model(n, nt, m, nz, nzp, a.titer)
+    s.free()
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "voxel-iter/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain ADMM solve, N=%d Nz=%d Nzp=%d ntheta=%d (Nrot=%d x %d) M=%d titer=%d"
+                   % (n, nz, nzp, nt, w["nrot"], w["ntheta_rot"], m, a.titer),
+                   "n": n, "nz": nz, "nzp": nzp, "ntheta": nt, "m": m, "titer": a.titer},
+        "iteration_hbm_frac": round(B / (ms * 1e-3) / HBM_PEAK, 4),
+        "iteration_model_bytes": B,
+        "roofline": roof,
+        "kernels": kern,
+    }
+    if not a.no_cpu:
+        print("bench: CPU baseline sample ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, a.titer)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -12,9 +12,9 @@ the (2n+2K)^2 padded grid with its periodic border, the checkerboard
 multiplications, the per-basis-function loops and the ghost/halo dual entries,
 so that the restructured GPU pipeline is checked against the literal algorithm.
 
-Pinning (what makes this oracle trustworthy, see tests/test_oracle_*.py):
+Pinning (what makes this oracle trustworthy, see tests/test_oracle_pins.py):
   * 1D/2D DFT conventions: checked against explicit DFT matrices built from the
-    definition (``test_oracle_fft.py``).
+    definition (the brute-force NUDFT test of ``test_oracle_pins.py``).
   * USFFT gridding (deapodisation, oversampled FFT, wrap, Gaussian gather):
     checked against a brute-force nonuniform DFT at the polar points, at the
     USFFT accuracy eps=1e-3, and against analytic line integrals of a Gaussian
@@ -488,7 +488,7 @@ def takephi(ntheta, m):
     phi_j[k] = exp(2 pi i (k - ntheta/2)(j - m/2)/ntheta)/sqrt(ntheta), stored as
     float32 [m, 2*ntheta] with re/im interleaved.
     """
-    x, y = np.meshgrid(np.arange(-(ntheta // 2), ntheta // 2), np.arange(-(m // 2), m // 2))
+    x, y = np.meshgrid(np.arange(ntheta) - ntheta // 2, np.arange(m) - m // 2)
     phi = np.zeros([m, 2 * ntheta], dtype='float32')
     phi[:, ::2] = np.cos(2 * np.pi * x * y / ntheta) / np.sqrt(ntheta)
     phi[:, 1::2] = np.sin(2 * np.pi * x * y / ntheta) / np.sqrt(ntheta)

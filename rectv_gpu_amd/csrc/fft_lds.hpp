@@ -20,6 +20,12 @@ struct FftPlan {
     int tw_off;  // offset of this size's twiddle table in the shared table buffer
 };
 
+// LDS position of element i of a transform row: one pad slot per 16 elements
+// breaks the power-of-two strides of the Stockham stores (bank conflicts).
+__host__ __device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
+// Row pitch (in float2) of a padded transform of length N.
+__host__ __device__ __forceinline__ int lp_stride(int N) { return N + (N >> 4) + 2; }
+
 template <bool INV>
 __device__ inline float2 mul_mi(float2 a) {  // a * (-i) forward, a * (+i) inverse
     return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
@@ -150,7 +156,7 @@ __device__ void fft_stage(float2 *buf, int stride, int T, int N, int Ns, int lgN
         const int j = jl + b * tpt;
         if (active && j < nb) {
 #pragma unroll
-            for (int r = 0; r < R; r++) v[b][r] = row[j + r * nb];
+            for (int r = 0; r < R; r++) v[b][r] = row[lp(j + r * nb)];
         }
     }
     __syncthreads();
@@ -169,9 +175,9 @@ __device__ void fft_stage(float2 *buf, int stride, int T, int N, int Ns, int lgN
                 }
             }
             dft<R, INV>(v[b]);
-            float2 *dst = row + jq * Ns * R + k;
+            const int d0 = jq * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[r * Ns] = v[b][r];
+            for (int r = 0; r < R; r++) row[lp(d0 + r * Ns)] = v[b][r];
         }
     }
     __syncthreads();
@@ -180,9 +186,12 @@ __device__ void fft_stage(float2 *buf, int stride, int T, int N, int Ns, int lgN
 // Threads per transform for length N (each thread holds at most E elements).
 __host__ __device__ inline int fft_tpt(int N, int E) { return (N + E - 1) / E; }
 
-// Transform T rows of length p.n held in LDS (buf[t*stride + i]).  Needs
+// Transform T rows of length p.n held in LDS (element i at buf[t*stride + lp(i)]).  Needs
 // blockDim.x >= T * fft_tpt(p.n, E).  Caller must __syncthreads() before.
-template <int E, bool INV>
+// ODD selects the instantiation that also carries radices 3, 5, 7 (needed only
+// for lengths with those factors; the power-of-two variant keeps register use
+// low enough to avoid spills).
+template <int E, bool INV, bool ODD>
 __device__ void fft_lds(float2 *buf, int stride, int T, const FftPlan &p, const float2 *__restrict__ twall,
                         int tid, int nthr) {
     (void)nthr;
@@ -197,10 +206,13 @@ __device__ void fft_lds(float2 *buf, int stride, int T, const FftPlan &p, const 
             case 8: fft_stage<8, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
             case 4: fft_stage<4, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
             case 2: fft_stage<2, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
-            case 3: fft_stage<3, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
-            case 5: fft_stage<5, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
-            case 7: fft_stage<7, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt); break;
-            default: break;
+            default:
+                if constexpr (ODD) {
+                    if (R == 3) fft_stage<3, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt);
+                    else if (R == 5) fft_stage<5, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt);
+                    else if (R == 7) fft_stage<7, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt);
+                }
+                break;
         }
         Ns *= R;
     }

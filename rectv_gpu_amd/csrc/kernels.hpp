@@ -19,12 +19,13 @@
 
 namespace rtv {
 
-constexpr int FFT_E = 32;       // complex elements per thread in the LDS FFT
-constexpr int FFT_THREADS = 512;
+constexpr int FFT_E = 16;       // complex elements per thread in the LDS FFT
+constexpr int FFT_THREADS = 1024;
 constexpr int TS = 16;          // spectral tile edge (cells) for gather/scatter
 constexpr int KW = 9;           // Gaussian window taps per axis, 2K+1 with K = 4
 constexpr int TH = TS + KW - 1; // tile + halo rows loaded by the gather
 constexpr int TP = TH + 1;      // LDS row pitch of the gather tile
+constexpr int FR_MAXT = 16;     // max basis functions per forward-row workgroup
 
 __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 
@@ -47,6 +48,7 @@ struct TileEntry {
 // Replaces makecomplex/decphi (kernels_operators.cuh:86-116) and divphi FWD +
 // the first half of the 2D cuFFT (kernels_radonusfft.cuh:4-24, radonusfft.cu:85-89).
 // One workgroup per (slice z, image row y, group of T basis functions).
+template <bool ODD>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float2 *__restrict__ phit,
            const float *__restrict__ deap, FftPlan plan, const float2 *__restrict__ tw, int n, int M, int T,
@@ -64,33 +66,45 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float2 *__
     const size_t plane = (size_t)n * n;
     const float *fz = f + (size_t)z * M * plane + (size_t)y * n;
     const float dy = deap[y];
-    for (int idx = tid; idx < Tc * n; idx += nthr) {
-        const int jt = idx / n, x = idx - jt * n;
-        const float2 *cj = phit + (size_t)(j0 + jt) * M;
-        float2 acc = make_float2(0.f, 0.f);
+    // each thread reads the M frames of its pixels once and forms all Tc u_j
+    for (int x = tid; x < n; x += nthr) {
+        float2 acc[FR_MAXT];
+#pragma unroll
+        for (int jt = 0; jt < FR_MAXT; jt++) acc[jt] = make_float2(0.f, 0.f);
         for (int i = 0; i < M; i++) {
             const float v = fz[(size_t)i * plane + x];
-            const float2 c = cj[i];  // phi_j[t_i]; use its conjugate
-            acc.x += v * c.x;
-            acc.y -= v * c.y;
+#pragma unroll
+            for (int jt = 0; jt < FR_MAXT; jt++) {
+                if (jt < Tc) {
+                    const float2 c = phit[(size_t)(j0 + jt) * M + i];  // phi_j[t_i]; use its conjugate
+                    acc[jt].x += v * c.x;
+                    acc[jt].y -= v * c.y;
+                }
+            }
         }
         const float s = deap[x] * dy * sgn(x + h);
-        lds[jt * stride + x + h] = cscale(acc, s);
         const int q = x < h ? x : x + n;  // zero padding
-        lds[jt * stride + q] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int jt = 0; jt < FR_MAXT; jt++) {
+            if (jt < Tc) {
+                lds[jt * stride + lp(x + h)] = cscale(acc[jt], s);
+                lds[jt * stride + lp(q)] = make_float2(0.f, 0.f);
+            }
+        }
     }
     __syncthreads();
-    fft_lds<FFT_E, false>(lds, stride, Tc, plan, tw, tid, nthr);
+    fft_lds<FFT_E, false, ODD>(lds, stride, Tc, plan, tw, tid, nthr);
     float2 *Hz = H + ((size_t)z * M + j0) * n * N + (size_t)y * N;
     for (int idx = tid; idx < Tc * N; idx += nthr) {
         const int jt = idx / N, kx = idx - jt * N;
-        Hz[(size_t)jt * n * N + kx] = cscale(lds[jt * stride + kx], sgn(kx));
+        Hz[(size_t)jt * n * N + kx] = cscale(lds[jt * stride + lp(kx)], sgn(kx));
     }
 }
 
 // F2: zero-padded column FFT -> oversampled spectrum G (second half of the 2D
 // cuFFT of radonusfft.cu:89 with the post-checkerboard).  One workgroup per
 // (z, j, tile of T adjacent columns).
+template <bool ODD>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, const float2 *__restrict__ tw,
            int n, int M, int T, int stride) {
@@ -105,16 +119,16 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
     const float2 *Hs = H + (size_t)zj * n * N + kx0;
     for (int idx = tid; idx < T * n; idx += nthr) {
         const int t = idx % T, y = idx / T;
-        lds[t * stride + y + h] = cscale(Hs[(size_t)y * N + t], sgn(y + h));
+        lds[t * stride + lp(y + h)] = cscale(Hs[(size_t)y * N + t], sgn(y + h));
         const int q = y < h ? y : y + n;
-        lds[t * stride + q] = make_float2(0.f, 0.f);
+        lds[t * stride + lp(q)] = make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft_lds<FFT_E, false>(lds, stride, T, plan, tw, tid, nthr);
+    fft_lds<FFT_E, false, ODD>(lds, stride, T, plan, tw, tid, nthr);
     float2 *Gs = G + (size_t)zj * N * N + kx0;
     for (int idx = tid; idx < T * N; idx += nthr) {
         const int t = idx % T, ky = idx / T;
-        Gs[(size_t)ky * N + t] = cscale(lds[t * stride + ky], sgn(ky));
+        Gs[(size_t)ky * N + t] = cscale(lds[t * stride + lp(ky)], sgn(ky));
     }
 }
 
@@ -200,6 +214,7 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
 //   ADJ: q[s] = post_a[s] * FFT(pre_a[s'] * h1[s'])  (radonusfft.cu:115-121 and
 //        the 1/n, s == 0 rule of the adjoint gather, kernels_radonusfft.cuh:102-108).
 enum { RAY_FWD = 1, RAY_SUBG = 2, RAY_ADJ = 4 };
+template <bool ODD>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restrict__ h1in,
       float *__restrict__ h1out, const float2 *__restrict__ pre_f, const float *__restrict__ post_f,
@@ -217,32 +232,32 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
             const int t = idx / n, s = idx - t * n;
             float2 v = make_float2(0.f, 0.f);
             if (s != 0) v = cmul(gq[(size_t)(r0 + t) * n + s], pre_f[s]);
-            lds[t * stride + s] = v;
+            lds[t * stride + lp(s)] = v;
         }
         __syncthreads();
-        fft_lds<FFT_E, true>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_lds<FFT_E, true, ODD>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
             const size_t o = (size_t)(r0 + t) * n + s;
-            float val = lds[t * stride + s].x * post_f[s];
+            float val = lds[t * stride + lp(s)].x * post_f[s];
             if (mode & RAY_SUBG) val -= g[o];
             acc += (double)val * val;
             if (h1out) h1out[o] = val;
-            lds[t * stride + s] = make_float2(val * pre_a[s], 0.f);
+            lds[t * stride + lp(s)] = make_float2(val * pre_a[s], 0.f);
         }
     } else {
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
-            lds[t * stride + s] = make_float2(h1in[(size_t)(r0 + t) * n + s] * pre_a[s], 0.f);
+            lds[t * stride + lp(s)] = make_float2(h1in[(size_t)(r0 + t) * n + s] * pre_a[s], 0.f);
         }
     }
     if (mode & RAY_ADJ) {
         __syncthreads();
-        fft_lds<FFT_E, false>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_lds<FFT_E, false, ODD>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
             float2 v = make_float2(0.f, 0.f);
-            if (s != 0) v = cmul(lds[t * stride + s], post_a[s]);
+            if (s != 0) v = cmul(lds[t * stride + lp(s)], post_a[s]);
             gq[(size_t)(r0 + t) * n + s] = v;
         }
     }
@@ -258,68 +273,89 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
     }
 }
 
-// A1: tile-owned Gaussian scatter (replaces gather ADJ with global atomicAdd,
-// kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62, run once per basis
-// function with mulphi(-1), operators.cu:47-56).  Each workgroup owns one
-// TS x TS tile of the 2n grid for all M spectra of slice z, accumulates every
-// sample whose window intersects the tile in LDS, then writes the tile once.
+// A1: tile-owned Gaussian scatter without atomics (replaces gather ADJ with
+// global atomicAdd, kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62, run
+// once per basis function after mulphi(-1), operators.cu:47-56).  One
+// workgroup owns one TS x TS tile of the 2n grid of slice z for all M spectra;
+// thread = cell.  The samples whose 9x9 window intersects the tile are staged in
+// LDS in chunks (window offset, 18 separable weights, q*conj(phi_j[k]) for all
+// j), then every thread walks the chunk and accumulates the samples covering
+// its cell in registers.  Fixed summation order: bitwise reproducible.
+constexpr int SC_CH = 256;  // samples staged per chunk (= threads)
+template <int MM>
 __global__ void __launch_bounds__(256)
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
           const int *__restrict__ ids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik,
           int n, int nt, int M, float beta, int ntiles) {
-    extern __shared__ float acc[];  // [M][TS*TS][2]
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    __shared__ float sw[SC_CH][2 * KW];
+    __shared__ int soff[SC_CH];
+    __shared__ float2 sp[SC_CH][MM];
+    const int tid = threadIdx.x;
     const int N = 2 * n, nT = N / TS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int e = L % ntiles;
     const int z = L / ntiles;
     const TileEntry te = tiles[e];
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
-    for (int idx = tid; idx < M * TS * TS * 2; idx += nthr) acc[idx] = 0.f;
-    __syncthreads();
+    const int cy = tid / TS, cx = tid - cy * TS;
     const float2 *q = gq + (size_t)z * nt * n;
-    for (int p = tid; p < te.count; p += nthr) {
-        const int id = ids[te.start + p];
-        const SampleGeom g = geo[id];
-        int dy = g.row0 - ty0;
-        if (dy < 0) dy += N;
-        const int ly0 = dy < TS ? dy : dy - N;
-        int dx = g.col0 - tx0;
-        if (dx < 0) dx += N;
-        const int lx0 = dx < TS ? dx : dx - N;
-        float wy[KW], wx[KW];
-        window_weights(g.fy, beta, wy);
-        window_weights(g.fx, beta, wx);
-        const int k = id / n;
-        const float2 qv = q[id];
-        const int a0 = max(0, -ly0), a1 = min(KW, TS - ly0);
-        const int b0 = max(0, -lx0), b1 = min(KW, TS - lx0);
-        for (int j = 0; j < M; j++) {
-            const float2 pj = cmulc(qv, phik[(size_t)k * M + j]);  // q * conj(phi_j[k])
-            float *aj = acc + (size_t)j * TS * TS * 2;
-            for (int a = a0; a < a1; a++) {
-                const float wa = wy[a];
-                float *row = aj + ((ly0 + a) * TS + lx0) * 2;
-                for (int b = b0; b < b1; b++) {
-                    const float w = wa * wx[b];
-                    atomicAdd(row + 2 * b, w * pj.x);
-                    atomicAdd(row + 2 * b + 1, w * pj.y);
+    float2 acc[MM];
+#pragma unroll
+    for (int j = 0; j < MM; j++) acc[j] = make_float2(0.f, 0.f);
+    for (int c0 = 0; c0 < te.count; c0 += SC_CH) {
+        const int cnt = min(SC_CH, te.count - c0);
+        if (c0 > 0) __syncthreads();
+        if (tid < cnt) {
+            const int id = ids[te.start + c0 + tid];
+            const SampleGeom g = geo[id];
+            int dy = g.row0 - ty0;
+            if (dy < 0) dy += N;
+            const int ly0 = dy < TS ? dy : dy - N;
+            int dx = g.col0 - tx0;
+            if (dx < 0) dx += N;
+            const int lx0 = dx < TS ? dx : dx - N;
+            soff[tid] = ((ly0 + 64) << 8) | (lx0 + 64);
+            float w[KW];
+            window_weights(g.fy, beta, w);
+#pragma unroll
+            for (int a = 0; a < KW; a++) sw[tid][a] = w[a];
+            window_weights(g.fx, beta, w);
+#pragma unroll
+            for (int a = 0; a < KW; a++) sw[tid][KW + a] = w[a];
+            const int k = id / n;
+            const float2 qv = q[id];
+            const float2 *ph = phik + (size_t)k * M;
+#pragma unroll
+            for (int j = 0; j < MM; j++)
+                if (j < M) sp[tid][j] = cmulc(qv, ph[j]);
+        }
+        __syncthreads();
+        for (int s = 0; s < cnt; s++) {
+            const int o = soff[s];
+            const int dy = cy - ((o >> 8) - 64);
+            const int dx = cx - ((o & 255) - 64);
+            if ((unsigned)dy < (unsigned)KW && (unsigned)dx < (unsigned)KW) {
+                const float w = sw[s][dy] * sw[s][KW + dx];
+#pragma unroll
+                for (int j = 0; j < MM; j++) {
+                    if (j < M) {
+                        const float2 pv = sp[s][j];
+                        acc[j].x += w * pv.x;
+                        acc[j].y += w * pv.y;
+                    }
                 }
             }
         }
     }
-    __syncthreads();
-    float2 *Gz = G + (size_t)z * M * N * N;
-    for (int idx = tid; idx < M * TS * TS; idx += nthr) {
-        const int j = idx / (TS * TS);
-        const int cell = idx - j * TS * TS;
-        const int r = cell / TS, c = cell - r * TS;
-        Gz[((size_t)j * N + ty0 + r) * N + tx0 + c] = make_float2(acc[2 * idx], acc[2 * idx + 1]);
-    }
+    float2 *Gz = G + (size_t)z * M * N * N + (size_t)(ty0 + cy) * N + tx0 + cx;
+#pragma unroll
+    for (int j = 0; j < MM; j++)
+        if (j < M) Gz[(size_t)j * N * N] = acc[j];
 }
 
 // A2: column inverse FFT of G keeping the n centre rows (cuFFT INVERSE of
 // radonusfft.cu:128 with the checkerboards, first half).
+template <bool ODD>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, const float2 *__restrict__ tw,
            int n, int M, int T, int stride) {
@@ -334,19 +370,20 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
     const float2 *Gs = G + (size_t)zj * N * N + kx0;
     for (int idx = tid; idx < T * N; idx += nthr) {
         const int t = idx % T, ky = idx / T;
-        lds[t * stride + ky] = cscale(Gs[(size_t)ky * N + t], sgn(ky));
+        lds[t * stride + lp(ky)] = cscale(Gs[(size_t)ky * N + t], sgn(ky));
     }
     __syncthreads();
-    fft_lds<FFT_E, true>(lds, stride, T, plan, tw, tid, nthr);
+    fft_lds<FFT_E, true, ODD>(lds, stride, T, plan, tw, tid, nthr);
     float2 *Hs = H + (size_t)zj * n * N + kx0;
     for (int idx = tid; idx < T * n; idx += nthr) {
         const int t = idx % T, y = idx / T;
-        Hs[(size_t)y * N + t] = cscale(lds[t * stride + y + h], sgn(y + h));
+        Hs[(size_t)y * N + t] = cscale(lds[t * stride + lp(y + h)], sgn(y + h));
     }
 }
 
 // A3: row inverse FFT keeping the n centre columns, deapodisation (divphi ADJ,
 // kernels_radonusfft.cuh:19-23).  One workgroup per (z, j, T rows).
+template <bool ODD>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__restrict__ deap, FftPlan plan,
            const float2 *__restrict__ tw, int n, int M, int T, int stride) {
@@ -362,15 +399,15 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
     const float2 *Hs = H + ((size_t)zj * n + y0) * N;
     for (int idx = tid; idx < Tc * N; idx += nthr) {
         const int t = idx / N, kx = idx - t * N;
-        lds[t * stride + kx] = cscale(Hs[(size_t)t * N + kx], sgn(kx));
+        lds[t * stride + lp(kx)] = cscale(Hs[(size_t)t * N + kx], sgn(kx));
     }
     __syncthreads();
-    fft_lds<FFT_E, true>(lds, stride, Tc, plan, tw, tid, nthr);
+    fft_lds<FFT_E, true, ODD>(lds, stride, Tc, plan, tw, tid, nthr);
     float2 *vs = v + ((size_t)zj * n + y0) * n;
     for (int idx = tid; idx < Tc * n; idx += nthr) {
         const int t = idx / n, x = idx - t * n;
         const float s = sgn(x + h) * deap[x] * deap[y0 + t];
-        vs[(size_t)t * n + x] = cscale(lds[t * stride + x + h], s);
+        vs[(size_t)t * n + x] = cscale(lds[t * stride + lp(x + h)], s);
     }
 }
 

@@ -211,6 +211,7 @@ struct Solver {
     int lds_max = 65536;
     // FFT
     FftPlan plan_n{}, plan_N{};
+    bool odd_n = false, odd_N = false;  // lengths with factors 3, 5, 7
     DBuf<float2> d_tw;
     int T_rows = 1, T_cols = 1, T_ray = 1, T_arows = 1, st_n = 0, st_N = 0, JC = 1;
     // state
@@ -262,6 +263,8 @@ struct Solver {
         N = 2 * n;
         RTV_REQUIRE(make_plan(n, plan_n) && make_plan(N, plan_N),
                     "n must factor into radices 2, 3, 5, 7 (FFT sizes n and 2n)");
+        odd_n = (n & (n - 1)) != 0;
+        odd_N = (N & (N - 1)) != 0;
         int ndev = 0;
         RTV_HIP(hipGetDeviceCount(&ndev));
         RTV_REQUIRE(ndev > 0, "no HIP device available");
@@ -299,6 +302,11 @@ struct Solver {
         d_part.alloc(3 * std::max(pb, rb) + 16, bytes);
         d_red.alloc(8, bytes);
         RTV_HIP(hipStreamSynchronize(st));
+        if (std::getenv("RECTV_VERBOSE"))
+            fprintf(stderr,
+                    "rectv: n=%d ntheta=%d m=%d nz=%d nzp=%d lds_max=%d T_rows=%d T_cols=%d T_ray=%d T_arows=%d JC=%d "
+                    "chunks/batch=%d device bytes=%zu\n",
+                    n, nt, M, nz, nzp, lds_max, T_rows, T_cols, T_ray, T_arows, JC, cb, bytes);
     }
 
     int fft_threads(int T, int len) const {
@@ -320,8 +328,8 @@ struct Solver {
         add(plan_n);
         add(plan_N);
         upload(d_tw, tw, bytes, st);
-        st_n = n + 1;
-        st_N = N + 1;
+        st_n = lp_stride(n);
+        st_N = lp_stride(N);
         // transforms per workgroup: bounded by LDS and by FFT_THREADS threads
         auto tmax = [&](int len, int stride) {
             return std::max(1, std::min(FFT_THREADS / fft_tpt(len, FFT_E), lds_max / (8 * stride)));
@@ -329,7 +337,7 @@ struct Solver {
         RTV_REQUIRE(fft_tpt(N, FFT_E) <= FFT_THREADS, "2n-point transform needs too many threads");
         const int tN = tmax(N, st_N), tn = tmax(n, st_n);
         RTV_REQUIRE(tN >= 1 && 8 * st_N <= lds_max, "2n-point transform does not fit in LDS");
-        T_rows = std::min(M, tN);
+        T_rows = std::min(std::min(M, tN), FR_MAXT);
         T_cols = 1;
         while (T_cols * 2 <= std::min(tN, 16) && N % (T_cols * 2) == 0) T_cols *= 2;
         T_arows = std::min(tN, 16);
@@ -338,13 +346,13 @@ struct Solver {
         const int np = (M + JC - 1) / JC;
         JC = (M + np - 1) / np;
         const int big = lds_max;
-        RTV_HIP(hipFuncSetAttribute((const void *)k_fwd_rows, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-        RTV_HIP(hipFuncSetAttribute((const void *)k_fwd_cols, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-        RTV_HIP(hipFuncSetAttribute((const void *)k_adj_cols, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-        RTV_HIP(hipFuncSetAttribute((const void *)k_adj_rows, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-        RTV_HIP(hipFuncSetAttribute((const void *)k_ray, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        const void *fns[] = {(const void *)k_fwd_rows<false>, (const void *)k_fwd_rows<true>,
+                             (const void *)k_fwd_cols<false>, (const void *)k_fwd_cols<true>,
+                             (const void *)k_adj_cols<false>, (const void *)k_adj_cols<true>,
+                             (const void *)k_adj_rows<false>, (const void *)k_adj_rows<true>,
+                             (const void *)k_ray<false>,      (const void *)k_ray<true>};
+        for (const void *fn : fns) RTV_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, big));
         RTV_HIP(hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-        RTV_HIP(hipFuncSetAttribute((const void *)k_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, big));
     }
 
     // ------------------------------------------------------------ profiling
@@ -436,7 +444,7 @@ struct Solver {
             const int thr = fft_threads(T_rows, N);
             const size_t lds = (size_t)T_rows * st_N * sizeof(float2);
             launch("fwd_rows", [&] {
-                hipLaunchKernelGGL(k_fwd_rows, dim3(blocks), dim3(thr), lds, st, f, d_H.p, d_phit.p, d_deap.p, plan_N,
+                hipLaunchKernelGGL(odd_N ? k_fwd_rows<true> : k_fwd_rows<false>, dim3(blocks), dim3(thr), lds, st, f, d_H.p, d_phit.p, d_deap.p, plan_N,
                                    d_tw.p, n, M, T_rows, st_N);
             });
         }
@@ -445,7 +453,7 @@ struct Solver {
             const int thr = fft_threads(T_cols, N);
             const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
             launch("fwd_cols", [&] {
-                hipLaunchKernelGGL(k_fwd_cols, dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p, n, M,
+                hipLaunchKernelGGL(odd_N ? k_fwd_cols<true> : k_fwd_cols<false>, dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p, n, M,
                                    T_cols, st_N);
             });
         }
@@ -467,7 +475,7 @@ struct Solver {
         const int thr = fft_threads(T_ray, n);
         const size_t lds = (size_t)T_ray * st_n * sizeof(float2);
         launch("ray", [&] {
-            hipLaunchKernelGGL(k_ray, dim3(blocks), dim3(thr), lds, st, d_gq.p, gdat, h1in, h1out, d_pre_f.p,
+            hipLaunchKernelGGL(odd_n ? k_ray<true> : k_ray<false>, dim3(blocks), dim3(thr), lds, st, d_gq.p, gdat, h1in, h1out, d_pre_f.p,
                                d_post_f.p, d_pre_a.p, d_post_a.p, plan_n, d_tw.p, n, nrays, T_ray, st_n, mode, partial);
         });
     }
@@ -476,10 +484,19 @@ struct Solver {
     void rays_to_images(int zc) {
         {
             const int blocks = na_tiles * zc;
-            const size_t lds = (size_t)M * TS * TS * 2 * sizeof(float);
             launch("scatter", [&] {
-                hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), lds, st, d_G.p, d_gq.p, d_geo.p, d_aids.p,
-                                   d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
+                if (M <= 4)
+                    hipLaunchKernelGGL(k_scatter<4>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p, d_aids.p,
+                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
+                else if (M <= 8)
+                    hipLaunchKernelGGL(k_scatter<8>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p, d_aids.p,
+                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
+                else if (M <= 16)
+                    hipLaunchKernelGGL(k_scatter<16>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                       d_aids.p, d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
+                else
+                    hipLaunchKernelGGL(k_scatter<32>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                       d_aids.p, d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
             });
         }
         {
@@ -487,7 +504,7 @@ struct Solver {
             const int thr = fft_threads(T_cols, N);
             const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
             launch("adj_cols", [&] {
-                hipLaunchKernelGGL(k_adj_cols, dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p, n, M,
+                hipLaunchKernelGGL(odd_N ? k_adj_cols<true> : k_adj_cols<false>, dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p, n, M,
                                    T_cols, st_N);
             });
         }
@@ -497,7 +514,7 @@ struct Solver {
             const int thr = fft_threads(T_arows, N);
             const size_t lds = (size_t)T_arows * st_N * sizeof(float2);
             launch("adj_rows", [&] {
-                hipLaunchKernelGGL(k_adj_rows, dim3(blocks), dim3(thr), lds, st, d_H.p, d_v.p, d_deap.p, plan_N,
+                hipLaunchKernelGGL(odd_N ? k_adj_rows<true> : k_adj_rows<false>, dim3(blocks), dim3(thr), lds, st, d_H.p, d_v.p, d_deap.p, plan_N,
                                    d_tw.p, n, M, T_arows, st_N);
             });
         }
