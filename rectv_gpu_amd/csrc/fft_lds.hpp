@@ -218,4 +218,76 @@ __device__ void fft_lds(float2 *buf, int stride, int T, const FftPlan &p, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Power-of-two lengths with the length known at compile time: radix-8 stages
+// (a final radix-4 or -2), every index computed from constants and shifts.
+// Threads per transform TPT = N / E; thread (t, jl) handles butterflies
+// j = jl + b * TPT.  Needs blockDim.x >= T * TPT.
+template <int R, int N, int Ns, int E, bool INV>
+__device__ __forceinline__ void p2_stage(float2 *row, bool active, int jl, const float2 *__restrict__ tw) {
+    constexpr int TPT = N / E;
+    constexpr int MB = E / R;
+    constexpr int nb = N / R;
+    constexpr int twstep = N / (Ns * R);
+    float2 v[MB][R];
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < MB; b++) {
+            const int j = jl + b * TPT;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = row[lp(j + r * nb)];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < MB; b++) {
+            const int j = jl + b * TPT;
+            const int k = j & (Ns - 1);
+            const int jq = j / Ns;
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    float2 w = tw[k * r * twstep];
+                    if (INV) w.y = -w.y;
+                    v[b][r] = cmul(v[b][r], w);
+                }
+            }
+            dft<R, INV>(v[b]);
+            const int d0 = jq * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) row[lp(d0 + r * Ns)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int Ns, int E, bool INV>
+__device__ __forceinline__ void p2_stages(float2 *row, bool active, int jl, const float2 *__restrict__ tw) {
+    if constexpr (Ns < N) {
+        constexpr int rem = N / Ns;
+        constexpr int R = rem >= 8 ? 8 : rem;
+        p2_stage<R, N, Ns, E, INV>(row, active, jl, tw);
+        p2_stages<N, Ns * R, E, INV>(row, active, jl, tw);
+    }
+}
+
+// Transform T rows of compile-time power-of-two length N in LDS.
+template <int N, int E, bool INV>
+__device__ __forceinline__ void fft_p2(float2 *buf, int stride, int T, const float2 *__restrict__ tw, int tid) {
+    static_assert((N & (N - 1)) == 0 && N >= E, "power-of-two length >= E");
+    constexpr int TPT = N / E;
+    const int t = tid / TPT, jl = tid & (TPT - 1);
+    p2_stages<N, 1, E, INV>(buf + t * stride, t < T, jl, tw);
+}
+
+// NP > 0: compile-time power-of-two length NP; NP == 0: runtime plan (radices
+// 2, 3, 4, 5, 7, 8).
+template <int NP, int E, bool INV>
+__device__ __forceinline__ void fft_any(float2 *buf, int stride, int T, const FftPlan &p, const float2 *__restrict__ twall,
+                                        int tid, int nthr) {
+    if constexpr (NP > 0) fft_p2<NP, E, INV>(buf, stride, T, twall + p.tw_off, tid);
+    else fft_lds<E, INV, true>(buf, stride, T, p, twall, tid, nthr);
+}
+
 }  // namespace rtv

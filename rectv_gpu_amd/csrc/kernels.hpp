@@ -48,12 +48,13 @@ struct TileEntry {
 // Replaces makecomplex/decphi (kernels_operators.cuh:86-116) and divphi FWD +
 // the first half of the 2D cuFFT (kernels_radonusfft.cuh:4-24, radonusfft.cu:85-89).
 // One workgroup per (slice z, image row y, group of T basis functions).
-template <bool ODD>
+template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float2 *__restrict__ phit,
            const float *__restrict__ deap, FftPlan plan, const float2 *__restrict__ tw, int n, int M, int T,
            int stride) {
     extern __shared__ float2 lds[];
+    if constexpr (NP > 0) n = NP / 2;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int ngr = (M + T - 1) / T;
@@ -62,7 +63,7 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float2 *__
     const int z = L / ngr / n;
     const int j0 = jg * T;
     const int Tc = min(T, M - j0);
-    const int N = 2 * n, h = n / 2;
+    const int N = NP ? NP : 2 * n, h = N / 4;
     const size_t plane = (size_t)n * n;
     const float *fz = f + (size_t)z * M * plane + (size_t)y * n;
     const float dy = deap[y];
@@ -93,7 +94,7 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float2 *__
         }
     }
     __syncthreads();
-    fft_lds<FFT_E, false, ODD>(lds, stride, Tc, plan, tw, tid, nthr);
+    fft_any<NP, FFT_E, false>(lds, stride, Tc, plan, tw, tid, nthr);
     float2 *Hz = H + ((size_t)z * M + j0) * n * N + (size_t)y * N;
     for (int idx = tid; idx < Tc * N; idx += nthr) {
         const int jt = idx / N, kx = idx - jt * N;
@@ -104,13 +105,14 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float2 *__
 // F2: zero-padded column FFT -> oversampled spectrum G (second half of the 2D
 // cuFFT of radonusfft.cu:89 with the post-checkerboard).  One workgroup per
 // (z, j, tile of T adjacent columns).
-template <bool ODD>
+template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, const float2 *__restrict__ tw,
            int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
+    if constexpr (NP > 0) n = NP / 2;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int N = 2 * n, h = n / 2;
+    const int N = NP ? NP : 2 * n, h = N / 4;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int ntile = N / T;
     const int kt = L % ntile;
@@ -124,7 +126,7 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
         lds[t * stride + lp(q)] = make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft_lds<FFT_E, false, ODD>(lds, stride, T, plan, tw, tid, nthr);
+    fft_any<NP, FFT_E, false>(lds, stride, T, plan, tw, tid, nthr);
     float2 *Gs = G + (size_t)zj * N * N + kx0;
     for (int idx = tid; idx < T * N; idx += nthr) {
         const int t = idx % T, ky = idx / T;
@@ -145,14 +147,27 @@ __device__ __forceinline__ void window_weights(float frac, float beta, float *w)
 
 // F3: tile-binned Gaussian gather from the M spectra of slice z, contracted with
 // phi_j[k] (replaces gather FWD kernels_radonusfft.cuh:81-144 run once per basis
-// function, plus mulphi/addreal's sum over j, operators.cu:22-31).  One workgroup
-// per (sample tile entry, z); the tile plus its 8-cell halo of JC spectra is
-// staged in LDS; each thread owns samples.  Output gq[z][k][s] (s >= 1).
+// function, plus mulphi/addreal's sum over j, operators.cu:22-31).
+// Work unit: a group of up to GSZ samples sharing one 9x9 window origin (rays
+// repeated every 2 pi, and rays theta+pi read in reverse detector order, land on
+// the same frequency cells).  One workgroup per (tile entry of <= GMAX groups,
+// slice z).  The group weights are staged once in LDS; the spectra tile plus its
+// 8-cell halo is staged JC basis functions at a time; lane = (group, j) with j
+// fastest, so each lane reads the window taps of one spectrum once for all group
+// members, and the contraction over j is a JC-lane shuffle reduction.
+constexpr int GMAX = 64;     // groups per gather workgroup
+constexpr int GJC = 4;       // spectra staged per pass (power of two)
+template <int GSZ>
 __global__ void __launch_bounds__(256)
 k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
-         const int *__restrict__ ids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik,
-         int n, int nt, int M, int JC, float beta, int nentries) {
-    extern __shared__ float2 lds[];
+         const int4 *__restrict__ groups, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik,
+         int n, int nt, int M, float beta, int nentries) {
+    __shared__ float2 tile[GJC][TH][TP];
+    __shared__ float wxs[GMAX][GSZ][KW];  // column weights per member
+    __shared__ float wys[GMAX][KW][GSZ];  // row weights, members contiguous per row
+    __shared__ float2 acc[GMAX][GSZ];
+    __shared__ int gofs[GMAX];
+    __shared__ int gid[GMAX][GSZ];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int N = 2 * n, nT = N / TS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -160,11 +175,39 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
     const int z = L / nentries;
     const TileEntry te = tiles[e];
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
+    const int cnt = te.count;
     const float2 *Gz = G + (size_t)z * M * N * N;
-    float2 *out = gq + (size_t)z * nt * n;
-    for (int jc = 0; jc < M; jc += JC) {
-        const int Jc = min(JC, M - jc);
-        if (jc > 0) __syncthreads();
+    // stage group geometry and separable weights
+    for (int t = tid; t < cnt * GSZ; t += nthr) {
+        const int g = t / GSZ, q = t - g * GSZ;
+        const int4 gr = groups[te.start + g];
+        const int id = q == 0 ? gr.x : q == 1 ? gr.y : q == 2 ? gr.z : gr.w;
+        gid[g][q] = id;
+        acc[g][q] = make_float2(0.f, 0.f);
+        if (q == 0) {
+            const SampleGeom g0 = geo[gr.x];
+            gofs[g] = (g0.row0 - ty0) * TP + (g0.col0 - tx0);
+        }
+        if (id >= 0) {
+            const SampleGeom gs = geo[id];
+            float w[KW];
+            window_weights(gs.fy, beta, w);
+#pragma unroll
+            for (int a = 0; a < KW; a++) wys[g][a][q] = w[a];
+            window_weights(gs.fx, beta, w);
+#pragma unroll
+            for (int a = 0; a < KW; a++) wxs[g][q][a] = w[a];
+        } else {
+#pragma unroll
+            for (int a = 0; a < KW; a++) {
+                wys[g][a][q] = 0.f;
+                wxs[g][q][a] = 0.f;
+            }
+        }
+    }
+    for (int jc = 0; jc < M; jc += GJC) {
+        const int Jc = min(GJC, M - jc);
+        __syncthreads();
         for (int idx = tid; idx < Jc * TH * TH; idx += nthr) {
             const int jl = idx / (TH * TH);
             const int rc = idx - jl * TH * TH;
@@ -173,37 +216,74 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
             if (gy >= N) gy -= N;
             int gx = tx0 + c;
             if (gx >= N) gx -= N;
-            lds[(jl * TH + r) * TP + c] = Gz[((size_t)(jc + jl) * N + gy) * N + gx];
+            tile[jl][r][c] = Gz[((size_t)(jc + jl) * N + gy) * N + gx];
         }
         __syncthreads();
-        for (int p = tid; p < te.count; p += nthr) {
-            const int id = ids[te.start + p];
-            const SampleGeom g = geo[id];
-            const int ly = g.row0 - ty0, lx = g.col0 - tx0;
-            float wy[KW], wx[KW];
-            window_weights(g.fy, beta, wy);
-            window_weights(g.fx, beta, wx);
-            const int k = id / n;
-            float2 acc = (jc == 0) ? make_float2(0.f, 0.f) : out[id];
-            for (int jl = 0; jl < Jc; jl++) {
-                const float2 *t0 = lds + (jl * TH + ly) * TP + lx;
-                float2 sum = make_float2(0.f, 0.f);
+        // items (g, jl), jl fastest; GJC divides 64 so a group's lanes share a wave
+        for (int it0 = 0; it0 < cnt * GJC; it0 += nthr) {
+            const int it = it0 + tid;
+            const int g = it / GJC, jl = it - (it / GJC) * GJC;
+            const bool live = it < cnt * GJC && jl < Jc;
+            float2 c[GSZ];
 #pragma unroll
+            for (int q = 0; q < GSZ; q++) c[q] = make_float2(0.f, 0.f);
+            if (live) {
+                float wx[GSZ][KW];
+#pragma unroll
+                for (int q = 0; q < GSZ; q++)
+#pragma unroll
+                    for (int b = 0; b < KW; b++) wx[q][b] = wxs[g][q][b];
+                const float2 *t0 = &tile[jl][0][0] + gofs[g];
+                float2 sum[GSZ];
+#pragma unroll
+                for (int q = 0; q < GSZ; q++) sum[q] = make_float2(0.f, 0.f);
+                // one window row per iteration keeps 9 taps live
+#pragma unroll 1
                 for (int a = 0; a < KW; a++) {
-                    float2 rs = make_float2(0.f, 0.f);
+                    float2 tap[KW];
 #pragma unroll
-                    for (int b = 0; b < KW; b++) {
-                        const float2 gv = t0[a * TP + b];
-                        rs.x += wx[b] * gv.x;
-                        rs.y += wx[b] * gv.y;
+                    for (int b = 0; b < KW; b++) tap[b] = t0[a * TP + b];
+                    float wy[GSZ];
+#pragma unroll
+                    for (int q = 0; q < GSZ; q++) wy[q] = wys[g][a][q];
+#pragma unroll
+                    for (int q = 0; q < GSZ; q++) {
+                        float2 rs = make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int b = 0; b < KW; b++) {
+                            rs.x += wx[q][b] * tap[b].x;
+                            rs.y += wx[q][b] * tap[b].y;
+                        }
+                        sum[q].x += wy[q] * rs.x;
+                        sum[q].y += wy[q] * rs.y;
                     }
-                    sum.x += wy[a] * rs.x;
-                    sum.y += wy[a] * rs.y;
                 }
-                acc = cadd(acc, cmul(phik[(size_t)k * M + jc + jl], sum));
+#pragma unroll
+                for (int q = 0; q < GSZ; q++) {
+                    const int id = gid[g][q];
+                    if (id >= 0) c[q] = cmul(phik[(size_t)(id / n) * M + jc + jl], sum[q]);
+                }
             }
-            out[id] = acc;
+            // sum the GJC basis functions of each group (lanes g*GJC .. g*GJC+GJC-1)
+#pragma unroll
+            for (int o = GJC / 2; o > 0; o >>= 1)
+#pragma unroll
+                for (int q = 0; q < GSZ; q++) {
+                    c[q].x += __shfl_xor(c[q].x, o, 64);
+                    c[q].y += __shfl_xor(c[q].y, o, 64);
+                }
+            if (it < cnt * GJC && jl == 0) {
+#pragma unroll
+                for (int q = 0; q < GSZ; q++) acc[g][q] = cadd(acc[g][q], c[q]);
+            }
         }
+    }
+    __syncthreads();
+    float2 *out = gq + (size_t)z * nt * n;
+    for (int t = tid; t < cnt * GSZ; t += nthr) {
+        const int g = t / GSZ, q = t - g * GSZ;
+        const int id = gid[g][q];
+        if (id >= 0) out[id] = acc[g][q];
     }
 }
 
@@ -214,7 +294,7 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
 //   ADJ: q[s] = post_a[s] * FFT(pre_a[s'] * h1[s'])  (radonusfft.cu:115-121 and
 //        the 1/n, s == 0 rule of the adjoint gather, kernels_radonusfft.cuh:102-108).
 enum { RAY_FWD = 1, RAY_SUBG = 2, RAY_ADJ = 4 };
-template <bool ODD>
+template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restrict__ h1in,
       float *__restrict__ h1out, const float2 *__restrict__ pre_f, const float *__restrict__ post_f,
@@ -222,6 +302,7 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
       const float2 *__restrict__ tw, int n, int nrays, int T, int stride, int mode,
       double *__restrict__ partial) {
     extern __shared__ float2 lds[];
+    if constexpr (NP > 0) n = NP;
     __shared__ double red[FFT_THREADS / 64];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int r0 = blockIdx.x * T;
@@ -235,7 +316,7 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
             lds[t * stride + lp(s)] = v;
         }
         __syncthreads();
-        fft_lds<FFT_E, true, ODD>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_any<NP, FFT_E, true>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
             const size_t o = (size_t)(r0 + t) * n + s;
@@ -253,7 +334,7 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
     }
     if (mode & RAY_ADJ) {
         __syncthreads();
-        fft_lds<FFT_E, false, ODD>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_any<NP, FFT_E, false>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
             float2 v = make_float2(0.f, 0.f);
@@ -273,23 +354,28 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
     }
 }
 
+constexpr int GS = 4;  // sample slots per origin group (int4 group records)
+
 // A1: tile-owned Gaussian scatter without atomics (replaces gather ADJ with
 // global atomicAdd, kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62, run
 // once per basis function after mulphi(-1), operators.cu:47-56).  One
-// workgroup owns one TS x TS tile of the 2n grid of slice z for all M spectra;
-// thread = cell.  The samples whose 9x9 window intersects the tile are staged in
-// LDS in chunks (window offset, 18 separable weights, q*conj(phi_j[k]) for all
-// j), then every thread walks the chunk and accumulates the samples covering
-// its cell in registers.  Fixed summation order: bitwise reproducible.
-constexpr int SC_CH = 256;  // samples staged per chunk (= threads)
+// workgroup owns one TS x TS tile of the 2n grid of slice z for all M spectra.
+// The origin groups whose 9x9 window intersects the tile are staged in LDS in
+// chunks (window offset, separable weights, q*conj(phi_j[k]) for every j).
+// Wave w owns tile rows 4w..4w+3; lane = (column cx, quarter jq of the basis
+// functions) and accumulates its cells in registers; a group is visited only for
+// the rows it covers (uniform branch), columns outside the window get weight 0.
+// Fixed summation order: bitwise reproducible.
+constexpr int SC_G = 64;  // groups staged per chunk (SC_G * GS = 256 = threads)
 template <int MM>
 __global__ void __launch_bounds__(256)
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
-          const int *__restrict__ ids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik,
+          const int4 *__restrict__ groups, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik,
           int n, int nt, int M, float beta, int ntiles) {
-    __shared__ float sw[SC_CH][2 * KW];
-    __shared__ int soff[SC_CH];
-    __shared__ float2 sp[SC_CH][MM];
+    constexpr int JPL = (MM + 3) / 4;  // basis functions per lane
+    __shared__ float sw[SC_G][GS][2 * KW];
+    __shared__ int soff[SC_G];
+    __shared__ float2 sp[SC_G][GS][MM];
     const int tid = threadIdx.x;
     const int N = 2 * n, nT = N / TS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -297,71 +383,105 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
     const int z = L / ntiles;
     const TileEntry te = tiles[e];
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
-    const int cy = tid / TS, cx = tid - cy * TS;
+    const int wv = tid >> 6, lane = tid & 63;
+    const int cx = lane & 15, jq = lane >> 4;
+    const int r0 = 4 * wv;
     const float2 *q = gq + (size_t)z * nt * n;
-    float2 acc[MM];
+    float2 acc[4][JPL];
 #pragma unroll
-    for (int j = 0; j < MM; j++) acc[j] = make_float2(0.f, 0.f);
-    for (int c0 = 0; c0 < te.count; c0 += SC_CH) {
-        const int cnt = min(SC_CH, te.count - c0);
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int jj = 0; jj < JPL; jj++) acc[i][jj] = make_float2(0.f, 0.f);
+    for (int c0 = 0; c0 < te.count; c0 += SC_G) {
+        const int cnt = min(SC_G, te.count - c0);
         if (c0 > 0) __syncthreads();
-        if (tid < cnt) {
-            const int id = ids[te.start + c0 + tid];
-            const SampleGeom g = geo[id];
-            int dy = g.row0 - ty0;
-            if (dy < 0) dy += N;
-            const int ly0 = dy < TS ? dy : dy - N;
-            int dx = g.col0 - tx0;
-            if (dx < 0) dx += N;
-            const int lx0 = dx < TS ? dx : dx - N;
-            soff[tid] = ((ly0 + 64) << 8) | (lx0 + 64);
-            float w[KW];
-            window_weights(g.fy, beta, w);
+        {
+            const int gi = tid / GS, si = tid - gi * GS;
+            if (gi < cnt) {
+                const int4 gr = groups[te.start + c0 + gi];
+                const int id = si == 0 ? gr.x : si == 1 ? gr.y : si == 2 ? gr.z : gr.w;
+                if (si == 0) {
+                    const SampleGeom g = geo[gr.x];
+                    int dy = g.row0 - ty0;
+                    if (dy < 0) dy += N;
+                    const int ly0 = dy < TS ? dy : dy - N;
+                    int dx = g.col0 - tx0;
+                    if (dx < 0) dx += N;
+                    const int lx0 = dx < TS ? dx : dx - N;
+                    soff[gi] = ((ly0 + 64) << 8) | (lx0 + 64);
+                }
+                if (id >= 0) {
+                    const SampleGeom g = geo[id];
+                    float w[KW];
+                    window_weights(g.fy, beta, w);
 #pragma unroll
-            for (int a = 0; a < KW; a++) sw[tid][a] = w[a];
-            window_weights(g.fx, beta, w);
+                    for (int a = 0; a < KW; a++) sw[gi][si][a] = w[a];
+                    window_weights(g.fx, beta, w);
 #pragma unroll
-            for (int a = 0; a < KW; a++) sw[tid][KW + a] = w[a];
-            const int k = id / n;
-            const float2 qv = q[id];
-            const float2 *ph = phik + (size_t)k * M;
+                    for (int a = 0; a < KW; a++) sw[gi][si][KW + a] = w[a];
+                    const int k = id / n;
+                    const float2 qv = q[id];
+                    const float2 *ph = phik + (size_t)k * M;
 #pragma unroll
-            for (int j = 0; j < MM; j++)
-                if (j < M) sp[tid][j] = cmulc(qv, ph[j]);
+                    for (int j = 0; j < MM; j++) sp[gi][si][j] = j < M ? cmulc(qv, ph[j]) : make_float2(0.f, 0.f);
+                } else {
+#pragma unroll
+                    for (int a = 0; a < 2 * KW; a++) sw[gi][si][a] = 0.f;
+#pragma unroll
+                    for (int j = 0; j < MM; j++) sp[gi][si][j] = make_float2(0.f, 0.f);
+                }
+            }
         }
         __syncthreads();
-        for (int s = 0; s < cnt; s++) {
-            const int o = soff[s];
-            const int dy = cy - ((o >> 8) - 64);
-            const int dx = cx - ((o & 255) - 64);
-            if ((unsigned)dy < (unsigned)KW && (unsigned)dx < (unsigned)KW) {
-                const float w = sw[s][dy] * sw[s][KW + dx];
+        for (int g = 0; g < cnt; g++) {
+            const int o = soff[g];
+            const int ly0 = (o >> 8) - 64, lx0 = (o & 255) - 64;
+            if (ly0 > r0 + 3 || ly0 + KW - 1 < r0) continue;  // window misses this wave's rows
+            const int dx = cx - lx0;
+            const bool vx = (unsigned)dx < (unsigned)KW;
+            const int dxc = vx ? dx : 0;
 #pragma unroll
-                for (int j = 0; j < MM; j++) {
-                    if (j < M) {
-                        const float2 pv = sp[s][j];
-                        acc[j].x += w * pv.x;
-                        acc[j].y += w * pv.y;
+            for (int si = 0; si < GS; si++) {
+                const float wxs = vx ? sw[g][si][KW + dxc] : 0.f;
+                float2 pv[JPL];
+#pragma unroll
+                for (int jj = 0; jj < JPL; jj++) pv[jj] = sp[g][si][jq * JPL + jj];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int dy = r0 + i - ly0;
+                    if ((unsigned)dy < (unsigned)KW) {
+                        const float w = sw[g][si][dy] * wxs;
+#pragma unroll
+                        for (int jj = 0; jj < JPL; jj++) {
+                            acc[i][jj].x += w * pv[jj].x;
+                            acc[i][jj].y += w * pv[jj].y;
+                        }
                     }
                 }
             }
         }
     }
-    float2 *Gz = G + (size_t)z * M * N * N + (size_t)(ty0 + cy) * N + tx0 + cx;
+    float2 *Gz = G + (size_t)z * M * N * N + (size_t)(ty0 + r0) * N + tx0 + cx;
 #pragma unroll
-    for (int j = 0; j < MM; j++)
-        if (j < M) Gz[(size_t)j * N * N] = acc[j];
+    for (int jj = 0; jj < JPL; jj++) {
+        const int j = jq * JPL + jj;
+        if (j < M) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) Gz[(size_t)j * N * N + (size_t)i * N] = acc[i][jj];
+        }
+    }
 }
 
 // A2: column inverse FFT of G keeping the n centre rows (cuFFT INVERSE of
 // radonusfft.cu:128 with the checkerboards, first half).
-template <bool ODD>
+template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, const float2 *__restrict__ tw,
            int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
+    if constexpr (NP > 0) n = NP / 2;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int N = 2 * n, h = n / 2;
+    const int N = NP ? NP : 2 * n, h = N / 4;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int ntile = N / T;
     const int kt = L % ntile;
@@ -373,7 +493,7 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
         lds[t * stride + lp(ky)] = cscale(Gs[(size_t)ky * N + t], sgn(ky));
     }
     __syncthreads();
-    fft_lds<FFT_E, true, ODD>(lds, stride, T, plan, tw, tid, nthr);
+    fft_any<NP, FFT_E, true>(lds, stride, T, plan, tw, tid, nthr);
     float2 *Hs = H + (size_t)zj * n * N + kx0;
     for (int idx = tid; idx < T * n; idx += nthr) {
         const int t = idx % T, y = idx / T;
@@ -383,13 +503,14 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
 
 // A3: row inverse FFT keeping the n centre columns, deapodisation (divphi ADJ,
 // kernels_radonusfft.cuh:19-23).  One workgroup per (z, j, T rows).
-template <bool ODD>
+template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__restrict__ deap, FftPlan plan,
            const float2 *__restrict__ tw, int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
+    if constexpr (NP > 0) n = NP / 2;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int N = 2 * n, h = n / 2;
+    const int N = NP ? NP : 2 * n, h = N / 4;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int nyt = (n + T - 1) / T;
     const int yt = L % nyt;
@@ -402,7 +523,7 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
         lds[t * stride + lp(kx)] = cscale(Hs[(size_t)t * N + kx], sgn(kx));
     }
     __syncthreads();
-    fft_lds<FFT_E, true, ODD>(lds, stride, Tc, plan, tw, tid, nthr);
+    fft_any<NP, FFT_E, true>(lds, stride, Tc, plan, tw, tid, nthr);
     float2 *vs = v + ((size_t)zj * n + y0) * n;
     for (int idx = tid; idx < Tc * n; idx += nthr) {
         const int t = idx / n, x = idx - t * n;

@@ -20,6 +20,28 @@ static thread_local std::string g_last_error;
 static const double PI_REF = 3.1415926535;  // kernels_radonusfft.cuh:1
 static const double PI_EXACT = 3.14159265358979323846;
 
+// ---------------------------------------------------------------- kernel pickers
+// FFT kernels are instantiated for power-of-two lengths (compile-time index
+// math) and once with a runtime plan (NP = 0) for every other length.
+#define RTV_PICKER(NAME, K)                                                                        \
+    static decltype(&K<0>) NAME(int len) {                                                         \
+        switch (len) {                                                                             \
+            case 64: return K<64>;                                                                 \
+            case 128: return K<128>;                                                               \
+            case 256: return K<256>;                                                               \
+            case 512: return K<512>;                                                               \
+            case 1024: return K<1024>;                                                             \
+            case 2048: return K<2048>;                                                             \
+            case 4096: return K<4096>;                                                             \
+            default: return K<0>;                                                                  \
+        }                                                                                          \
+    }
+RTV_PICKER(pick_fwd_rows, k_fwd_rows)
+RTV_PICKER(pick_fwd_cols, k_fwd_cols)
+RTV_PICKER(pick_adj_cols, k_adj_cols)
+RTV_PICKER(pick_adj_rows, k_adj_rows)
+RTV_PICKER(pick_ray, k_ray)
+
 // ---------------------------------------------------------------- FFT plans
 static bool make_plan(int N, FftPlan &p) {
     p.n = N;
@@ -38,11 +60,12 @@ static bool make_plan(int N, FftPlan &p) {
 // ---------------------------------------------------------------- geometry
 struct HostGeom {
     std::vector<SampleGeom> geo;
-    std::vector<int> fids, aids;
+    std::vector<int4> fgroups, agroups;
     std::vector<TileEntry> ftiles, atiles;
     std::vector<float2> pre_f, post_a, phik, phit;
     std::vector<float> post_f, pre_a, deap;
     float beta = 0.f;
+    int fgs = 4;  // samples per gather group
 };
 
 // USFFT constants, radonusfft.cu:10-12: mu = -log(eps)/(2 n^2) held in float.
@@ -86,54 +109,80 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             ftile[id] = (g.row0 / TS) * nT + g.col0 / TS;
         }
     }
-    // forward lists: each sample in the tile holding its window origin
+    // Work lists.  Samples are grouped by window origin (up to GS per group, ids
+    // ascending) within each tile; forward: the tile holding the origin (split
+    // into entries of at most chunk_max groups); adjoint: every tile the 9x9
+    // window intersects (one entry per tile, empty tiles included).
+    struct Key {
+        unsigned long long k;  // (tile << 32) | origin
+        int id;
+        bool operator<(const Key &o) const { return k != o.k ? k < o.k : id < o.id; }
+    };
     const int ntile = nT * nT;
+    auto make_groups = [&](std::vector<Key> &keys, std::vector<int4> &grp, std::vector<TileEntry> &ent,
+                           bool all_tiles, int split, int gsz) {
+        std::sort(keys.begin(), keys.end());
+        grp.clear();
+        ent.clear();
+        std::vector<int> gstart(ntile + 1, 0);
+        size_t i = 0;
+        for (int t = 0; t < ntile; t++) {
+            gstart[t] = (int)grp.size();
+            while (i < keys.size() && (int)(keys[i].k >> 32) == t) {
+                size_t j = i;
+                while (j < keys.size() && keys[j].k == keys[i].k) j++;
+                for (size_t a = i; a < j; a += gsz) {
+                    int v[GS];
+                    for (int q = 0; q < GS; q++) v[q] = (q < gsz && a + q < j) ? keys[a + q].id : -1;
+                    grp.push_back(make_int4(v[0], v[1], v[2], v[3]));
+                }
+                i = j;
+            }
+        }
+        gstart[ntile] = (int)grp.size();
+        for (int t = 0; t < ntile; t++) {
+            const int c = gstart[t + 1] - gstart[t];
+            if (c == 0) {
+                if (all_tiles) ent.push_back(TileEntry{t, gstart[t], 0, 0});
+                continue;
+            }
+            for (int s0 = 0; s0 < c; s0 += split)
+                ent.push_back(TileEntry{t, gstart[t] + s0, std::min(split, c - s0), 0});
+        }
+        std::stable_sort(ent.begin(), ent.end(), [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
+    };
     {
-        std::vector<int> cnt(ntile, 0), off(ntile + 1, 0);
-        for (size_t id = 0; id < ns; id++)
-            if (ftile[id] >= 0) cnt[ftile[id]]++;
-        for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
-        hg.fids.assign(off[ntile], 0);
-        std::vector<int> pos(off.begin(), off.end() - 1);
-        for (size_t id = 0; id < ns; id++)
-            if (ftile[id] >= 0) hg.fids[pos[ftile[id]]++] = (int)id;
-        hg.ftiles.clear();
-        for (int t = 0; t < ntile; t++)
-            for (int s0 = 0; s0 < cnt[t]; s0 += chunk_max)
-                hg.ftiles.push_back(TileEntry{t, off[t] + s0, std::min(chunk_max, cnt[t] - s0), 0});
-        std::stable_sort(hg.ftiles.begin(), hg.ftiles.end(),
-                         [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
+        std::vector<Key> keys;
+        keys.reserve(ns);
+        for (size_t id = 0; id < ns; id++) {
+            if (ftile[id] < 0) continue;
+            const SampleGeom &g = hg.geo[id];
+            keys.push_back(Key{((unsigned long long)ftile[id] << 32) | (unsigned)(g.row0 * N + g.col0), (int)id});
+        }
+        // group size for the gather: 4 when windows are shared widely (repeated
+        // rotations), else smaller so that empty slots do not waste work
+        std::sort(keys.begin(), keys.end());
+        size_t origins = 0;
+        for (size_t a = 0; a < keys.size(); a++)
+            if (a == 0 || keys[a].k != keys[a - 1].k) origins++;
+        const double share = origins ? (double)keys.size() / origins : 1.0;
+        hg.fgs = share >= 3.0 ? 4 : (share >= 1.6 ? 2 : 1);
+        make_groups(keys, hg.fgroups, hg.ftiles, false, chunk_max, hg.fgs);
     }
-    // adjoint lists: each sample in every tile its 9x9 window intersects
     {
-        std::vector<int> cnt(ntile, 0);
-        auto tiles_of = [&](const SampleGeom &g, int *out) {
+        std::vector<Key> keys;
+        keys.reserve(3 * ns);
+        for (size_t id = 0; id < ns; id++) {
+            if (ftile[id] < 0) continue;
+            const SampleGeom &g = hg.geo[id];
             const int ty[2] = {g.row0 / TS, ((g.row0 + KW - 1) % N) / TS};
             const int tx[2] = {g.col0 / TS, ((g.col0 + KW - 1) % N) / TS};
-            int c = 0;
+            const unsigned org = (unsigned)(g.row0 * N + g.col0);
             for (int a = 0; a < (ty[1] != ty[0] ? 2 : 1); a++)
-                for (int b = 0; b < (tx[1] != tx[0] ? 2 : 1); b++) out[c++] = ty[a] * nT + tx[b];
-            return c;
-        };
-        int tl[4];
-        for (size_t id = 0; id < ns; id++) {
-            if (ftile[id] < 0) continue;
-            const int c = tiles_of(hg.geo[id], tl);
-            for (int q = 0; q < c; q++) cnt[tl[q]]++;
+                for (int b = 0; b < (tx[1] != tx[0] ? 2 : 1); b++)
+                    keys.push_back(Key{((unsigned long long)(ty[a] * nT + tx[b]) << 32) | org, (int)id});
         }
-        std::vector<int> off(ntile + 1, 0);
-        for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
-        hg.aids.assign(off[ntile], 0);
-        std::vector<int> pos(off.begin(), off.end() - 1);
-        for (size_t id = 0; id < ns; id++) {
-            if (ftile[id] < 0) continue;
-            const int c = tiles_of(hg.geo[id], tl);
-            for (int q = 0; q < c; q++) hg.aids[pos[tl[q]]++] = (int)id;
-        }
-        hg.atiles.resize(ntile);
-        for (int t = 0; t < ntile; t++) hg.atiles[t] = TileEntry{t, off[t], cnt[t], 0};
-        std::stable_sort(hg.atiles.begin(), hg.atiles.end(),
-                         [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
+        make_groups(keys, hg.agroups, hg.atiles, true, 1 << 30, GS);
     }
     // per-ray tables (takeshift kernels_radonusfft.cuh:191-198 with c = -/+(center - n/2),
     // fftshift signs, 1/n, PI/mu, 1/sqrt(n ntheta), ntheta/sqrt(M))
@@ -228,11 +277,11 @@ struct Solver {
     // geometry
     bool inputs = false;
     DBuf<SampleGeom> d_geo;
-    DBuf<int> d_fids, d_aids;
+    DBuf<int4> d_fgroups, d_agroups;
     DBuf<TileEntry> d_ftiles, d_atiles;
     DBuf<float2> d_phik, d_phit, d_pre_f, d_post_a;
     DBuf<float> d_post_f, d_pre_a, d_deap;
-    int nf_entries = 0, na_tiles = 0;
+    int nf_entries = 0, na_tiles = 0, fgs = 4;
     float beta = 0.f;
     // profiling
     bool prof = false;
@@ -346,13 +395,9 @@ struct Solver {
         const int np = (M + JC - 1) / JC;
         JC = (M + np - 1) / np;
         const int big = lds_max;
-        const void *fns[] = {(const void *)k_fwd_rows<false>, (const void *)k_fwd_rows<true>,
-                             (const void *)k_fwd_cols<false>, (const void *)k_fwd_cols<true>,
-                             (const void *)k_adj_cols<false>, (const void *)k_adj_cols<true>,
-                             (const void *)k_adj_rows<false>, (const void *)k_adj_rows<true>,
-                             (const void *)k_ray<false>,      (const void *)k_ray<true>};
+        const void *fns[] = {(const void *)pick_fwd_rows(N), (const void *)pick_fwd_cols(N),
+                             (const void *)pick_adj_cols(N), (const void *)pick_adj_rows(N), (const void *)pick_ray(n)};
         for (const void *fn : fns) RTV_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-        RTV_HIP(hipFuncSetAttribute((const void *)k_gather, hipFuncAttributeMaxDynamicSharedMemorySize, big));
     }
 
     // ------------------------------------------------------------ profiling
@@ -402,12 +447,13 @@ struct Solver {
         for (int k = 0; k < nt; k++) RTV_REQUIRE(std::isfinite(theta[k]), "theta contains non-finite values");
         RTV_REQUIRE(std::isfinite(center), "center must be finite");
         HostGeom hg;
-        build_geometry(hg, n, nt, M, theta, phi, center, 1024);
+        build_geometry(hg, n, nt, M, theta, phi, center, GMAX);
+        fgs = hg.fgs;
         beta = hg.beta;
         upload(d_geo, hg.geo, bytes, st);
-        upload(d_fids, hg.fids, bytes, st);
+        upload(d_fgroups, hg.fgroups, bytes, st);
         upload(d_ftiles, hg.ftiles, bytes, st);
-        upload(d_aids, hg.aids, bytes, st);
+        upload(d_agroups, hg.agroups, bytes, st);
         upload(d_atiles, hg.atiles, bytes, st);
         upload(d_phik, hg.phik, bytes, st);
         upload(d_phit, hg.phit, bytes, st);
@@ -444,7 +490,7 @@ struct Solver {
             const int thr = fft_threads(T_rows, N);
             const size_t lds = (size_t)T_rows * st_N * sizeof(float2);
             launch("fwd_rows", [&] {
-                hipLaunchKernelGGL(odd_N ? k_fwd_rows<true> : k_fwd_rows<false>, dim3(blocks), dim3(thr), lds, st, f, d_H.p, d_phit.p, d_deap.p, plan_N,
+                hipLaunchKernelGGL(pick_fwd_rows(N), dim3(blocks), dim3(thr), lds, st, f, d_H.p, d_phit.p, d_deap.p, plan_N,
                                    d_tw.p, n, M, T_rows, st_N);
             });
         }
@@ -453,17 +499,23 @@ struct Solver {
             const int thr = fft_threads(T_cols, N);
             const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
             launch("fwd_cols", [&] {
-                hipLaunchKernelGGL(odd_N ? k_fwd_cols<true> : k_fwd_cols<false>, dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p, n, M,
+                hipLaunchKernelGGL(pick_fwd_cols(N), dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p, n, M,
                                    T_cols, st_N);
             });
         }
         {
             const int blocks = nf_entries * zc;
-            const size_t lds = (size_t)JC * TH * TP * sizeof(float2);
             if (blocks > 0)
                 launch("gather", [&] {
-                    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), lds, st, d_G.p, d_gq.p, d_geo.p, d_fids.p,
-                                       d_ftiles.p, d_phik.p, n, nt, M, JC, beta, nf_entries);
+                    if (fgs == 4)
+                        hipLaunchKernelGGL(k_gather<4>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                           d_fgroups.p, d_ftiles.p, d_phik.p, n, nt, M, beta, nf_entries);
+                    else if (fgs == 2)
+                        hipLaunchKernelGGL(k_gather<2>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                           d_fgroups.p, d_ftiles.p, d_phik.p, n, nt, M, beta, nf_entries);
+                    else
+                        hipLaunchKernelGGL(k_gather<1>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                           d_fgroups.p, d_ftiles.p, d_phik.p, n, nt, M, beta, nf_entries);
                 });
         }
         ray(zc, gdat, nullptr, h1out, mode, partial);
@@ -475,7 +527,7 @@ struct Solver {
         const int thr = fft_threads(T_ray, n);
         const size_t lds = (size_t)T_ray * st_n * sizeof(float2);
         launch("ray", [&] {
-            hipLaunchKernelGGL(odd_n ? k_ray<true> : k_ray<false>, dim3(blocks), dim3(thr), lds, st, d_gq.p, gdat, h1in, h1out, d_pre_f.p,
+            hipLaunchKernelGGL(pick_ray(n), dim3(blocks), dim3(thr), lds, st, d_gq.p, gdat, h1in, h1out, d_pre_f.p,
                                d_post_f.p, d_pre_a.p, d_post_a.p, plan_n, d_tw.p, n, nrays, T_ray, st_n, mode, partial);
         });
     }
@@ -486,17 +538,17 @@ struct Solver {
             const int blocks = na_tiles * zc;
             launch("scatter", [&] {
                 if (M <= 4)
-                    hipLaunchKernelGGL(k_scatter<4>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p, d_aids.p,
+                    hipLaunchKernelGGL(k_scatter<4>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p, d_agroups.p,
                                        d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
                 else if (M <= 8)
-                    hipLaunchKernelGGL(k_scatter<8>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p, d_aids.p,
+                    hipLaunchKernelGGL(k_scatter<8>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p, d_agroups.p,
                                        d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
                 else if (M <= 16)
                     hipLaunchKernelGGL(k_scatter<16>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                       d_aids.p, d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
+                                       d_agroups.p, d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
                 else
                     hipLaunchKernelGGL(k_scatter<32>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                       d_aids.p, d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
+                                       d_agroups.p, d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
             });
         }
         {
@@ -504,7 +556,7 @@ struct Solver {
             const int thr = fft_threads(T_cols, N);
             const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
             launch("adj_cols", [&] {
-                hipLaunchKernelGGL(odd_N ? k_adj_cols<true> : k_adj_cols<false>, dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p, n, M,
+                hipLaunchKernelGGL(pick_adj_cols(N), dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p, n, M,
                                    T_cols, st_N);
             });
         }
@@ -514,7 +566,7 @@ struct Solver {
             const int thr = fft_threads(T_arows, N);
             const size_t lds = (size_t)T_arows * st_N * sizeof(float2);
             launch("adj_rows", [&] {
-                hipLaunchKernelGGL(odd_N ? k_adj_rows<true> : k_adj_rows<false>, dim3(blocks), dim3(thr), lds, st, d_H.p, d_v.p, d_deap.p, plan_N,
+                hipLaunchKernelGGL(pick_adj_rows(N), dim3(blocks), dim3(thr), lds, st, d_H.p, d_v.p, d_deap.p, plan_N,
                                    d_tw.p, n, M, T_arows, st_N);
             });
         }
@@ -764,8 +816,8 @@ struct Solver {
         d_part.release(t);
         d_red.release(t);
         d_geo.release(t);
-        d_fids.release(t);
-        d_aids.release(t);
+        d_fgroups.release(t);
+        d_agroups.release(t);
         d_ftiles.release(t);
         d_atiles.release(t);
         d_phik.release(t);
