@@ -356,118 +356,135 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
 
 constexpr int GS = 4;  // sample slots per origin group (int4 group records)
 
-// A1: tile-owned Gaussian scatter without atomics (replaces gather ADJ with
-// global atomicAdd, kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62, run
-// once per basis function after mulphi(-1), operators.cu:47-56).  One
-// workgroup owns one TS x TS tile of the 2n grid of slice z for all M spectra.
-// The origin groups whose 9x9 window intersects the tile are staged in LDS in
-// chunks (window offset, separable weights, q*conj(phi_j[k]) for every j).
-// Wave w owns tile rows 4w..4w+3; lane = (column cx, quarter jq of the basis
-// functions) and accumulates its cells in registers; a group is visited only for
-// the rows it covers (uniform branch), columns outside the window get weight 0.
-// Fixed summation order: bitwise reproducible.
-constexpr int SC_G = 64;  // groups staged per chunk (SC_G * GS = 256 = threads)
-template <int MM>
-__global__ void __launch_bounds__(256)
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// A1: tile-owned Gaussian scatter on the FP32 matrix cores (replaces gather ADJ
+// with global atomicAdd, kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62,
+// run once per basis function after mulphi(-1), operators.cu:47-56).
+// One workgroup (16 waves) owns one TS x TS tile of the 2n grid for ZW slices;
+// wave r owns tile row r.  The samples whose 9x9 window intersects the tile are
+// listed sorted by window row.  A chunk of them is staged in LDS once for the ZW
+// slices: 16-wide weight rows wyr[s][r] = wy[r-ly0], wxr[s][i] = wx[i-lx0]
+// (zero outside the window) and, per slice, P[s][col] = q*conj(phi_j[k]) split
+// into real (col = j) and imaginary (col = M + j) columns.  For its row the wave
+// takes the run of samples covering it and accumulates
+//   D[cell i][col] += sum_k wyr[k][r] * wxr[k][i] * P[k][col]
+// with v_mfma_f32_16x16x4f32 (16 cells x 16 columns x 4 samples per
+// instruction).  No atomics; fixed summation order.
+constexpr int SM_CH = 128;  // samples staged per chunk (8 staging threads each)
+constexpr int SM_ZW = 2;    // slices per workgroup
+template <int NB>           // 16-column blocks: 2*M <= 16*NB
+__global__ void __launch_bounds__(1024)
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
-          const int4 *__restrict__ groups, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik,
-          int n, int nt, int M, float beta, int ntiles) {
-    constexpr int JPL = (MM + 3) / 4;  // basis functions per lane
-    __shared__ float sw[SC_G][GS][2 * KW];
-    __shared__ int soff[SC_G];
-    __shared__ float2 sp[SC_G][GS][MM];
+          const int *__restrict__ sids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik, int n,
+          int nt, int M, float beta, int ntiles, int nzb) {
+    constexpr int NC = 16 * NB;
+    constexpr int NCP = NC + 1;  // padded row of P
+    constexpr int NBIN = TS + KW - 1;  // window rows ly0 in [-8, TS)
+    __shared__ float wyr[SM_CH][TS];
+    __shared__ float wxr[SM_CH][TS];
+    __shared__ float sP[SM_ZW][SM_CH][NCP];
+    __shared__ int sly[SM_CH];
+    __shared__ int bfirst[NBIN], blast[NBIN];
+    static_assert(SM_ZW * SM_CH * NCP >= 16 * 16 * NC, "output transpose must fit in the P buffer");
     const int tid = threadIdx.x;
     const int N = 2 * n, nT = N / TS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int e = L % ntiles;
-    const int z = L / ntiles;
+    const int z0 = (L / ntiles) * SM_ZW;
+    const int zc = min(SM_ZW, nzb - z0);
     const TileEntry te = tiles[e];
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
-    const int wv = tid >> 6, lane = tid & 63;
-    const int cx = lane & 15, jq = lane >> 4;
-    const int r0 = 4 * wv;
-    const float2 *q = gq + (size_t)z * nt * n;
-    float2 acc[4][JPL];
+    const int r = tid >> 6, l = tid & 63;
+    const int li = l & 15, lk = l >> 4;
+    const int sm = tid >> 3, part = tid & 7;  // staging: sample, eighth
+    v4f acc[SM_ZW][NB];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int zz = 0; zz < SM_ZW; zz++)
 #pragma unroll
-        for (int jj = 0; jj < JPL; jj++) acc[i][jj] = make_float2(0.f, 0.f);
-    for (int c0 = 0; c0 < te.count; c0 += SC_G) {
-        const int cnt = min(SC_G, te.count - c0);
-        if (c0 > 0) __syncthreads();
-        {
-            const int gi = tid / GS, si = tid - gi * GS;
-            if (gi < cnt) {
-                const int4 gr = groups[te.start + c0 + gi];
-                const int id = si == 0 ? gr.x : si == 1 ? gr.y : si == 2 ? gr.z : gr.w;
-                if (si == 0) {
-                    const SampleGeom g = geo[gr.x];
-                    int dy = g.row0 - ty0;
-                    if (dy < 0) dy += N;
-                    const int ly0 = dy < TS ? dy : dy - N;
-                    int dx = g.col0 - tx0;
-                    if (dx < 0) dx += N;
-                    const int lx0 = dx < TS ? dx : dx - N;
-                    soff[gi] = ((ly0 + 64) << 8) | (lx0 + 64);
+        for (int b = 0; b < NB; b++) acc[zz][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < te.count; c0 += SM_CH) {
+        const int cnt = min(SM_CH, te.count - c0);
+        __syncthreads();
+        if (tid < NBIN) {
+            bfirst[tid] = cnt;
+            blast[tid] = 0;
+        }
+        if (sm < cnt) {
+            const int id = sids[te.start + c0 + sm];
+            const SampleGeom g = geo[id];
+            int dy = g.row0 - ty0;
+            if (dy < 0) dy += N;
+            int dx = g.col0 - tx0;
+            if (dx < 0) dx += N;
+            const int ly0 = dy < TS ? dy : dy - N;
+            const int lx0 = dx < TS ? dx : dx - N;
+            if (part == 0) sly[sm] = ly0;
+            // weight-table entries 2*part, 2*part+1 (zero outside the window)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int i = 2 * part + h;
+                const int ay = i - ly0, ax = i - lx0;
+                const float dyv = (float)(ay - 4) - g.fy, dxv = (float)(ax - 4) - g.fx;
+                wyr[sm][i] = ((unsigned)ay < (unsigned)KW) ? expf(-beta * dyv * dyv) : 0.f;
+                wxr[sm][i] = ((unsigned)ax < (unsigned)KW) ? expf(-beta * dxv * dxv) : 0.f;
+            }
+            // P columns for j = 2*part, 2*part+1 (and the zero padding)
+            const float2 *ph = phik + (size_t)(id / n) * M;
+            for (int zz = 0; zz < zc; zz++) {
+                const float2 qv = gq[(size_t)(z0 + zz) * nt * n + id];
+                for (int j = part; j < M; j += 8) {
+                    const float2 p = cmulc(qv, ph[j]);  // q * conj(phi_j[k])
+                    sP[zz][sm][j] = p.x;
+                    sP[zz][sm][M + j] = p.y;
                 }
-                if (id >= 0) {
-                    const SampleGeom g = geo[id];
-                    float w[KW];
-                    window_weights(g.fy, beta, w);
-#pragma unroll
-                    for (int a = 0; a < KW; a++) sw[gi][si][a] = w[a];
-                    window_weights(g.fx, beta, w);
-#pragma unroll
-                    for (int a = 0; a < KW; a++) sw[gi][si][KW + a] = w[a];
-                    const int k = id / n;
-                    const float2 qv = q[id];
-                    const float2 *ph = phik + (size_t)k * M;
-#pragma unroll
-                    for (int j = 0; j < MM; j++) sp[gi][si][j] = j < M ? cmulc(qv, ph[j]) : make_float2(0.f, 0.f);
-                } else {
-#pragma unroll
-                    for (int a = 0; a < 2 * KW; a++) sw[gi][si][a] = 0.f;
-#pragma unroll
-                    for (int j = 0; j < MM; j++) sp[gi][si][j] = make_float2(0.f, 0.f);
-                }
+                for (int c = 2 * M + part; c < NC; c += 8) sP[zz][sm][c] = 0.f;
             }
         }
         __syncthreads();
-        for (int g = 0; g < cnt; g++) {
-            const int o = soff[g];
-            const int ly0 = (o >> 8) - 64, lx0 = (o & 255) - 64;
-            if (ly0 > r0 + 3 || ly0 + KW - 1 < r0) continue;  // window misses this wave's rows
-            const int dx = cx - lx0;
-            const bool vx = (unsigned)dx < (unsigned)KW;
-            const int dxc = vx ? dx : 0;
+        // first/last index of each window-row bin (the list is sorted by ly0)
+        if (tid < cnt) {
+            const int v = sly[tid];
+            if (tid == 0 || sly[tid - 1] != v) bfirst[v + KW - 1] = tid;
+            if (tid == cnt - 1 || sly[tid + 1] != v) blast[v + KW - 1] = tid + 1;
+        }
+        __syncthreads();
+        // samples covering row r: bins ly0 = r-8 .. r
+        int lo = cnt, hi = 0;
 #pragma unroll
-            for (int si = 0; si < GS; si++) {
-                const float wxs = vx ? sw[g][si][KW + dxc] : 0.f;
-                float2 pv[JPL];
+        for (int q = 0; q < KW; q++) {
+            lo = min(lo, bfirst[r + q]);
+            hi = max(hi, blast[r + q]);
+        }
+        for (int k0 = lo; k0 < hi; k0 += 4) {
+            const int s = k0 + lk;
+            const bool ok = s < hi;
+            const int sc = ok ? s : lo;
+            const float av = ok ? wyr[sc][r] * wxr[sc][li] : 0.f;
 #pragma unroll
-                for (int jj = 0; jj < JPL; jj++) pv[jj] = sp[g][si][jq * JPL + jj];
+            for (int zz = 0; zz < SM_ZW; zz++) {
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int dy = r0 + i - ly0;
-                    if ((unsigned)dy < (unsigned)KW) {
-                        const float w = sw[g][si][dy] * wxs;
-#pragma unroll
-                        for (int jj = 0; jj < JPL; jj++) {
-                            acc[i][jj].x += w * pv[jj].x;
-                            acc[i][jj].y += w * pv[jj].y;
-                        }
-                    }
+                for (int b = 0; b < NB; b++) {
+                    const float bv = sP[zz][sc][b * 16 + li];
+                    acc[zz][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[zz][b], 0, 0, 0);
                 }
             }
         }
     }
-    float2 *Gz = G + (size_t)z * M * N * N + (size_t)(ty0 + r0) * N + tx0 + cx;
+    // acc[zz][b][v] = D_zz[cell 4*lk + v][col 16*b + li]; transpose through LDS, write rows
+    float *wl = &sP[0][0][0] + (size_t)r * 16 * NC;  // [cell][col] of this wave's row
+    float2 *Gt = G + (size_t)(ty0 + r) * N + tx0;
+    for (int zz = 0; zz < zc; zz++) {
+        __syncthreads();
 #pragma unroll
-    for (int jj = 0; jj < JPL; jj++) {
-        const int j = jq * JPL + jj;
-        if (j < M) {
+        for (int b = 0; b < NB; b++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) Gz[(size_t)j * N * N + (size_t)i * N] = acc[i][jj];
+            for (int v = 0; v < 4; v++) wl[(4 * lk + v) * NC + 16 * b + li] = acc[zz][b][v];
+        __syncthreads();
+        float2 *Gr = Gt + (size_t)(z0 + zz) * M * N * N;
+        for (int idx = l; idx < M * 16; idx += 64) {
+            const int j = idx >> 4, i = idx & 15;
+            Gr[(size_t)j * N * N + i] = make_float2(wl[i * NC + j], wl[i * NC + M + j]);
         }
     }
 }

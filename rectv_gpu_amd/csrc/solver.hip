@@ -60,7 +60,8 @@ static bool make_plan(int N, FftPlan &p) {
 // ---------------------------------------------------------------- geometry
 struct HostGeom {
     std::vector<SampleGeom> geo;
-    std::vector<int4> fgroups, agroups;
+    std::vector<int4> fgroups;
+    std::vector<int> asids;
     std::vector<TileEntry> ftiles, atiles;
     std::vector<float2> pre_f, post_a, phik, phit;
     std::vector<float> post_f, pre_a, deap;
@@ -149,7 +150,8 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             for (int s0 = 0; s0 < c; s0 += split)
                 ent.push_back(TileEntry{t, gstart[t] + s0, std::min(split, c - s0), 0});
         }
-        std::stable_sort(ent.begin(), ent.end(), [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
+        if (std::getenv("RECTV_DENSE_FIRST"))
+            std::stable_sort(ent.begin(), ent.end(), [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
     };
     {
         std::vector<Key> keys;
@@ -169,6 +171,8 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         hg.fgs = share >= 3.0 ? 4 : (share >= 1.6 ? 2 : 1);
         make_groups(keys, hg.fgroups, hg.ftiles, false, chunk_max, hg.fgs);
     }
+    // adjoint lists: every tile the 9x9 window intersects, each tile's samples
+    // sorted by (local window row, local window column, id)
     {
         std::vector<Key> keys;
         keys.reserve(3 * ns);
@@ -177,12 +181,27 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             const SampleGeom &g = hg.geo[id];
             const int ty[2] = {g.row0 / TS, ((g.row0 + KW - 1) % N) / TS};
             const int tx[2] = {g.col0 / TS, ((g.col0 + KW - 1) % N) / TS};
-            const unsigned org = (unsigned)(g.row0 * N + g.col0);
             for (int a = 0; a < (ty[1] != ty[0] ? 2 : 1); a++)
-                for (int b = 0; b < (tx[1] != tx[0] ? 2 : 1); b++)
-                    keys.push_back(Key{((unsigned long long)(ty[a] * nT + tx[b]) << 32) | org, (int)id});
+                for (int b = 0; b < (tx[1] != tx[0] ? 2 : 1); b++) {
+                    int dy = (g.row0 - ty[a] * TS + N) % N, dx = (g.col0 - tx[b] * TS + N) % N;
+                    const int ly0 = dy < TS ? dy : dy - N, lx0 = dx < TS ? dx : dx - N;
+                    const unsigned loc = (unsigned)((ly0 + 64) << 8 | (lx0 + 64));
+                    keys.push_back(Key{((unsigned long long)(ty[a] * nT + tx[b]) << 32) | loc, (int)id});
+                }
         }
-        make_groups(keys, hg.agroups, hg.atiles, true, 1 << 30, GS);
+        std::sort(keys.begin(), keys.end());
+        hg.asids.resize(keys.size());
+        std::vector<int> cnt(ntile, 0), off(ntile + 1, 0);
+        for (size_t a = 0; a < keys.size(); a++) {
+            hg.asids[a] = keys[a].id;
+            cnt[keys[a].k >> 32]++;
+        }
+        for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
+        hg.atiles.resize(ntile);
+        for (int t = 0; t < ntile; t++) hg.atiles[t] = TileEntry{t, off[t], cnt[t], 0};
+        if (std::getenv("RECTV_DENSE_FIRST"))
+            std::stable_sort(hg.atiles.begin(), hg.atiles.end(),
+                             [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
     }
     // per-ray tables (takeshift kernels_radonusfft.cuh:191-198 with c = -/+(center - n/2),
     // fftshift signs, 1/n, PI/mu, 1/sqrt(n ntheta), ntheta/sqrt(M))
@@ -277,7 +296,8 @@ struct Solver {
     // geometry
     bool inputs = false;
     DBuf<SampleGeom> d_geo;
-    DBuf<int4> d_fgroups, d_agroups;
+    DBuf<int4> d_fgroups;
+    DBuf<int> d_asids;
     DBuf<TileEntry> d_ftiles, d_atiles;
     DBuf<float2> d_phik, d_phit, d_pre_f, d_post_a;
     DBuf<float> d_post_f, d_pre_a, d_deap;
@@ -453,7 +473,7 @@ struct Solver {
         upload(d_geo, hg.geo, bytes, st);
         upload(d_fgroups, hg.fgroups, bytes, st);
         upload(d_ftiles, hg.ftiles, bytes, st);
-        upload(d_agroups, hg.agroups, bytes, st);
+        upload(d_asids, hg.asids, bytes, st);
         upload(d_atiles, hg.atiles, bytes, st);
         upload(d_phik, hg.phik, bytes, st);
         upload(d_phit, hg.phit, bytes, st);
@@ -535,20 +555,17 @@ struct Solver {
     // adjoint: d_gq (q samples of zc slices) -> d_v images
     void rays_to_images(int zc) {
         {
-            const int blocks = na_tiles * zc;
+            const int blocks = na_tiles * ((zc + SM_ZW - 1) / SM_ZW);
             launch("scatter", [&] {
-                if (M <= 4)
-                    hipLaunchKernelGGL(k_scatter<4>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p, d_agroups.p,
-                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
-                else if (M <= 8)
-                    hipLaunchKernelGGL(k_scatter<8>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p, d_agroups.p,
-                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
-                else if (M <= 16)
-                    hipLaunchKernelGGL(k_scatter<16>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                       d_agroups.p, d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
+                if (2 * M <= 16)
+                    hipLaunchKernelGGL(k_scatter<1>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
+                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles, zc);
+                else if (2 * M <= 32)
+                    hipLaunchKernelGGL(k_scatter<2>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
+                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles, zc);
                 else
-                    hipLaunchKernelGGL(k_scatter<32>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                       d_agroups.p, d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles);
+                    hipLaunchKernelGGL(k_scatter<4>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
+                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles, zc);
             });
         }
         {
@@ -817,7 +834,7 @@ struct Solver {
         d_red.release(t);
         d_geo.release(t);
         d_fgroups.release(t);
-        d_agroups.release(t);
+        d_asids.release(t);
         d_ftiles.release(t);
         d_atiles.release(t);
         d_phik.release(t);
