@@ -75,6 +75,45 @@ __device__ inline void dft8(float2 *v) {
     }
 }
 
+// 16-point DFT as 4 x 4: inner DFT-4 over x[n2 + 4 n1], twiddle W16^(n2 k1),
+// outer DFT-4 over n2; output X[k1 + 4 k2].
+template <bool INV>
+__device__ inline void dft16(float2 *v) {
+    float2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        float2 t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+        dft4<INV>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) y[n2][k1] = t[k1];
+    }
+    // W16^m, m = n2*k1 in {0,1,2,3,4,6,9}
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, r2 = 0.70710678118654752440f;
+#pragma unroll
+    for (int n2 = 1; n2 < 4; n2++) {
+#pragma unroll
+        for (int k1 = 1; k1 < 4; k1++) {
+            const int m = n2 * k1;
+            float cr, ci;  // exp(-2 pi i m / 16)
+            if (m == 1) { cr = c1; ci = -s1; }
+            else if (m == 2) { cr = r2; ci = -r2; }
+            else if (m == 3) { cr = s1; ci = -c1; }
+            else if (m == 4) { cr = 0.f; ci = -1.f; }
+            else if (m == 6) { cr = -r2; ci = -r2; }
+            else { cr = -c1; ci = s1; }  // m == 9: exp(-2 pi i 9/16)
+            if (INV) ci = -ci;
+            y[n2][k1] = cmul(y[n2][k1], make_float2(cr, ci));
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        float2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        dft4<INV>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = t[k2];
+    }
+}
+
 // Direct odd-radix DFT: y_k = sum_r x_r exp(-/+ 2 pi i r k / R).
 // cos/sin(2 pi m / R) for m < R as literals (folded once the loops are unrolled).
 template <int R>
@@ -137,6 +176,7 @@ __device__ inline void dft(float2 *v) {
     if constexpr (R == 2) dft2<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v);
     else if constexpr (R == 8) dft8<INV>(v);
+    else if constexpr (R == 16) dft16<INV>(v);
     else dft_odd<R, INV>(v);
 }
 
@@ -219,8 +259,8 @@ __device__ void fft_lds(float2 *buf, int stride, int T, const FftPlan &p, const 
 }
 
 // ---------------------------------------------------------------------------
-// Power-of-two lengths with the length known at compile time: radix-8 stages
-// (a final radix-4 or -2), every index computed from constants and shifts.
+// Power-of-two lengths with the length known at compile time: radix-16 stages
+// (a final radix-8, -4 or -2), every index computed from constants and shifts.
 // Threads per transform TPT = N / E; thread (t, jl) handles butterflies
 // j = jl + b * TPT.  Needs blockDim.x >= T * TPT.
 template <int R, int N, int Ns, int E, bool INV>
@@ -266,7 +306,7 @@ template <int N, int Ns, int E, bool INV>
 __device__ __forceinline__ void p2_stages(float2 *row, bool active, int jl, const float2 *__restrict__ tw) {
     if constexpr (Ns < N) {
         constexpr int rem = N / Ns;
-        constexpr int R = rem >= 8 ? 8 : rem;
+        constexpr int R = (rem >= 16 && E >= 16) ? 16 : (rem >= 8 ? 8 : rem);
         p2_stage<R, N, Ns, E, INV>(row, active, jl, tw);
         p2_stages<N, Ns * R, E, INV>(row, active, jl, tw);
     }

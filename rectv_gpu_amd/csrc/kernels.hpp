@@ -29,6 +29,15 @@ constexpr int FR_MAXT = 16;     // max basis functions per forward-row workgroup
 
 __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 
+// Transforms per FFT workgroup for a compile-time power-of-two length L:
+// FFT_WG / (L / FFT_E) rows, at most 16 (the host uses the same rule).  FFT_WG
+// = 512 threads keeps LDS near 70 KB so that two workgroups share a CU and one
+// transforms while the other loads or stores.
+constexpr int FFT_WG = 512;
+__host__ __device__ constexpr int fft_rows_p2(int L) {
+    return (FFT_WG / (L / FFT_E)) < 1 ? 1 : ((FFT_WG / (L / FFT_E)) < 16 ? (FFT_WG / (L / FFT_E)) : 16);
+}
+
 struct SampleGeom {
     int row0;   // first window row in the 2n grid (already mod 2n)
     int col0;   // first window column
@@ -111,6 +120,7 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
            int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
     if constexpr (NP > 0) n = NP / 2;
+    if constexpr (NP > 0) T = fft_rows_p2(NP);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int N = NP ? NP : 2 * n, h = N / 4;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -303,6 +313,7 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
       double *__restrict__ partial) {
     extern __shared__ float2 lds[];
     if constexpr (NP > 0) n = NP;
+    if constexpr (NP > 0) T = fft_rows_p2(NP);
     __shared__ double red[FFT_THREADS / 64];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int r0 = blockIdx.x * T;
@@ -497,6 +508,7 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
            int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
     if constexpr (NP > 0) n = NP / 2;
+    if constexpr (NP > 0) T = fft_rows_p2(NP);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int N = NP ? NP : 2 * n, h = N / 4;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -526,6 +538,7 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
            const float2 *__restrict__ tw, int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
     if constexpr (NP > 0) n = NP / 2;
+    if constexpr (NP > 0) T = fft_rows_p2(NP);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int N = NP ? NP : 2 * n, h = N / 4;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -590,7 +603,9 @@ __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i
 //   out = fm - 0.5*div(rho*(grad fm - psi) + mu) - 0.5*Re sum_j v_j phi_j[t_i]
 // Thread per (chunk, y, x) column marching over the chunk's slices and frames.
 // Halo slabs (chunk neighbours) come from A, the previous outer iterate
-// (rectv.cu:170-171; clamped borders kernels_operators.cuh:4-20).
+// (rectv.cu:170-171; clamped borders kernels_operators.cuh:4-20).  The dual
+// values needed by the backward differences in z and t are the previous
+// slice's / frame's own values, so they are carried instead of reloaded.
 template <int MM>
 __global__ void __launch_bounds__(256)
 k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__restrict__ out,
@@ -607,19 +622,16 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
     const size_t col = (size_t)y * n + x;
     const int xm = x > 0 ? x - 1 : x, xp = x < n - 1 ? x + 1 : x;
     const int ym = y > 0 ? y - 1 : y, yp = y < n - 1 ? y + 1 : y;
-    float fz[MM], fzm[MM], fzp[MM], pw[MM], mw[MM];
-#pragma unroll
-    for (int i = 0; i < MM; i++)
-        if (i < M) fz[i] = fm[zb * S + i * plane + col];
-#pragma unroll
-    for (int i = 0; i < MM; i++)
-        if (i < M) fzm[i] = flgl ? A[(zb - 1) * S + i * plane + col] : fz[i];
+    const float r = a.rho;
+    float fz[MM], fzp[MM], hzc[MM], rp[MM];
+    // own column at slice 0 and the z-gradient dual values on plane 0
 #pragma unroll
     for (int i = 0; i < MM; i++) {
         if (i < M) {
+            fz[i] = fm[zb * S + i * plane + col];
+            const float fzm = flgl ? A[(zb - 1) * S + i * plane + col] : fz[i];
             const size_t d = dual_idx(a, c, 0, i + 1, y + 1, x + 1);
-            pw[i] = psi[d].w;
-            mw[i] = mu[d].w;
+            hzc[i] = r * (a.gs * (fz[i] - fzm) - psi[d].w) + mu[d].w;
         }
     }
     for (int zl = 0; zl < nzp; zl++) {
@@ -631,15 +643,22 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
                 else fzp[i] = flgr ? A[(zg + 1) * S + i * plane + col] : fz[i];
             }
         }
-        float2 vr[MM];
+        // R^* part: rp_i = Re sum_j v_j phi_j[t_i]
+#pragma unroll
+        for (int i = 0; i < MM; i++) rp[i] = 0.f;
         if (v) {
             const float2 *vz = v + ((size_t)(zg - zv0) * M) * plane + col;
+            for (int j = 0; j < M; j++) {
+                const float2 vj = vz[(size_t)j * plane];
+                const float2 *cj = phit + (size_t)j * M;
 #pragma unroll
-            for (int j = 0; j < MM; j++)
-                if (j < M) vr[j] = vz[(size_t)j * plane];
+                for (int i = 0; i < MM; i++)
+                    if (i < M) rp[i] += vj.x * cj[i].x - vj.y * cj[i].y;
+            }
         }
+        // t-gradient dual value on frame plane 0: E(t'=1) - E(t'=0) = 0
         const size_t d0 = dual_idx(a, c, zl + 1, 0, y + 1, x + 1);
-        float pt = psi[d0].z, mt = mu[d0].z;
+        float htc = r * (0.f - psi[d0].z) + mu[d0].z;
         const float *fl = fm + zg * S;
 #pragma unroll
         for (int i = 0; i < MM; i++) {
@@ -648,49 +667,27 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
                 const float *fi = fl + i * plane;
                 const float fxp = fi[(size_t)y * n + xp], fxm = fi[(size_t)y * n + xm];
                 const float fyp = fi[(size_t)yp * n + x], fym = fi[(size_t)ym * n + x];
-                // register indices must stay compile-time constants (i is unrolled)
+                // register indices stay compile-time constants (i is unrolled)
                 const float ftp = (i + 1 < M) ? fz[(i + 1 < MM) ? i + 1 : i] : f0;
-                const float ftm = (i > 0) ? fz[(i > 0) ? i - 1 : 0] : f0;
                 const size_t d = dual_idx(a, c, zl + 1, i + 1, y + 1, x + 1);
                 const float4 ps = psi[d], ms = mu[d];
                 const float px = psi[d - 1].x, mx = mu[d - 1].x;
                 const float py = psi[d - (n + 1)].y, my = mu[d - (n + 1)].y;
-                const float r = a.rho;
                 const float hx0 = r * (a.gs * (fxp - f0) - ps.x) + ms.x;
                 const float hy0 = r * (a.gs * (fyp - f0) - ps.y) + ms.y;
                 const float ht0 = r * (a.gt * (ftp - f0) - ps.z) + ms.z;
                 const float hz0 = r * (a.gs * (fzp[i] - f0) - ps.w) + ms.w;
                 const float hxm = r * (a.gs * (f0 - fxm) - px) + mx;
                 const float hym = r * (a.gs * (f0 - fym) - py) + my;
-                const float htm = r * (a.gt * (f0 - ftm) - pt) + mt;
-                const float hzm = r * (a.gs * (f0 - fzm[i]) - pw[i]) + mw[i];
-                const float dv = a.ks * (hxm - hx0) + a.ks * (hym - hy0) + a.kt * (htm - ht0) + a.ks * (hzm - hz0);
-                float val = f0 - 0.5f * dv;
-                if (v) {
-                    float s = 0.f;
-#pragma unroll
-                    for (int j = 0; j < MM; j++) {
-                        if (j < M) {
-                            const float2 cj = phit[(size_t)j * M + i];
-                            s += vr[j].x * cj.x - vr[j].y * cj.y;
-                        }
-                    }
-                    val -= 0.5f * s;
-                }
-                out[zg * S + i * plane + col] = val;
-                pt = ps.z;
-                mt = ms.z;
-                pw[i] = ps.w;
-                mw[i] = ms.w;
+                const float dv = a.ks * (hxm - hx0) + a.ks * (hym - hy0) + a.kt * (htc - ht0) + a.ks * (hzc[i] - hz0);
+                out[zg * S + i * plane + col] = f0 - 0.5f * dv - 0.5f * rp[i];
+                htc = ht0;
+                hzc[i] = hz0;
             }
         }
 #pragma unroll
-        for (int i = 0; i < MM; i++) {
-            if (i < M) {
-                fzm[i] = fz[i];
-                fz[i] = fzp[i];
-            }
-        }
+        for (int i = 0; i < MM; i++)
+            if (i < M) fz[i] = fzp[i];
     }
 }
 
@@ -740,7 +737,8 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
             bool hn, hc;
             const size_t sn = slab(zq + 1, hn);
             const size_t sc0 = slab(zq, hc);
-            float xF[MM], xD[MM], yF[MM], yD[MM];
+            const size_t ox = sc0 * S + (size_t)yc * n + xn;
+            const size_t oy = sc0 * S + (size_t)yn * n + xc;
 #pragma unroll
             for (int i = 0; i < MM; i++) {
                 if (i < M) {
@@ -748,14 +746,6 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
                     const float an = A[on];
                     nF[i] = hn ? an : fm[on];
                     nD[i] = hn ? 0.f : an - nF[i];
-                    const size_t ox = sc0 * S + i * plane + (size_t)yc * n + xn;
-                    const float ax = A[ox];
-                    xF[i] = hc ? ax : fm[ox];
-                    xD[i] = hc ? 0.f : ax - xF[i];
-                    const size_t oy = sc0 * S + i * plane + (size_t)yn * n + xc;
-                    const float ay = A[oy];
-                    yF[i] = hc ? ay : fm[oy];
-                    yD[i] = hc ? 0.f : ay - yF[i];
                 }
             }
 #pragma unroll
@@ -765,12 +755,16 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
                     const int ic = iq > 0 ? iq - 1 : 0;
                     const int it = iq < MM ? iq : MM - 1;
                     const float e0 = cF[ic];
-                    const float hx = a.gs * (xF[ic] - e0);
-                    const float hy = a.gs * (yF[ic] - e0);
+                    const float axv = A[ox + ic * plane], ayv = A[oy + ic * plane];
+                    const float xFv = hc ? axv : fm[ox + ic * plane];
+                    const float yFv = hc ? ayv : fm[oy + ic * plane];
+                    const float xDv = hc ? 0.f : axv - xFv, yDv = hc ? 0.f : ayv - yFv;
+                    const float hx = a.gs * (xFv - e0);
+                    const float hy = a.gs * (yFv - e0);
                     const float ht = a.gt * ((iq < M ? cF[it] : e0) - e0);
                     const float hz = a.gs * (nF[ic] - e0);
                     const float d0 = cD[ic];
-                    const float dx = a.gs * (xD[ic] - d0), dy = a.gs * (yD[ic] - d0);
+                    const float dx = a.gs * (xDv - d0), dy = a.gs * (yDv - d0);
                     const float dt = a.gt * ((iq < M ? cD[it] : d0) - d0), dz = a.gs * (nD[ic] - d0);
                     ss += (double)dx * dx + (double)dy * dy + (double)dt * dt + (double)dz * dz;
                     if (zq >= 1 && iq >= 1 && yq >= 1 && xq >= 1) sc += (double)d0 * d0;

@@ -401,7 +401,7 @@ struct Solver {
         st_N = lp_stride(N);
         // transforms per workgroup: bounded by LDS and by FFT_THREADS threads
         auto tmax = [&](int len, int stride) {
-            return std::max(1, std::min(FFT_THREADS / fft_tpt(len, FFT_E), lds_max / (8 * stride)));
+            return std::max(1, std::min(std::max(1, FFT_WG / fft_tpt(len, FFT_E)), lds_max / (8 * stride)));
         };
         RTV_REQUIRE(fft_tpt(N, FFT_E) <= FFT_THREADS, "2n-point transform needs too many threads");
         const int tN = tmax(N, st_N), tn = tmax(n, st_n);
@@ -411,6 +411,15 @@ struct Solver {
         while (T_cols * 2 <= std::min(tN, 16) && N % (T_cols * 2) == 0) T_cols *= 2;
         T_arows = std::min(tN, 16);
         T_ray = std::min(tn, 16);
+        // power-of-two lengths use compile-time row counts inside the kernels
+        if ((N & (N - 1)) == 0 && N >= 64 && N <= 4096) {
+            T_cols = T_arows = fft_rows_p2(N);
+            RTV_REQUIRE(8 * st_N * T_cols <= lds_max, "LDS too small for the 2n-point transforms");
+        }
+        if ((n & (n - 1)) == 0 && n >= 64 && n <= 4096) {
+            T_ray = fft_rows_p2(n);
+            RTV_REQUIRE(8 * st_n * T_ray <= lds_max, "LDS too small for the n-point transforms");
+        }
         JC = std::max(1, std::min(M, lds_max / (TH * TP * 8)));
         const int np = (M + JC - 1) / JC;
         JC = (M + np - 1) / np;
