@@ -217,8 +217,13 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
     }
     for (int jc = 0; jc < M; jc += GJC) {
         const int Jc = min(GJC, M - jc);
-        __syncthreads();
-        for (int idx = tid; idx < Jc * TH * TH; idx += nthr) {
+        // stage the tile: every global load of a thread is issued before its LDS
+        // stores (fixed trip count, unrolled) so the memory latency is paid once
+        constexpr int NLD = (GJC * TH * TH + 255) / 256;
+        float2 stv[NLD];
+#pragma unroll
+        for (int q = 0; q < NLD; q++) {
+            const int idx = tid + q * 256;
             const int jl = idx / (TH * TH);
             const int rc = idx - jl * TH * TH;
             const int r = rc / TH, c = rc - r * TH;
@@ -226,7 +231,18 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
             if (gy >= N) gy -= N;
             int gx = tx0 + c;
             if (gx >= N) gx -= N;
-            tile[jl][r][c] = Gz[((size_t)(jc + jl) * N + gy) * N + gx];
+            stv[q] = (idx < GJC * TH * TH && jl < Jc) ? Gz[((size_t)(jc + jl) * N + gy) * N + gx]
+                                                      : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NLD; q++) {
+            const int idx = tid + q * 256;
+            if (idx < GJC * TH * TH) {
+                const int jl = idx / (TH * TH);
+                const int rc = idx - jl * TH * TH;
+                tile[jl][rc / TH][rc - (rc / TH) * TH] = stv[q];
+            }
         }
         __syncthreads();
         // items (g, jl), jl fastest; GJC divides 64 so a group's lanes share a wave

@@ -406,7 +406,10 @@ struct Solver {
         RTV_REQUIRE(fft_tpt(N, FFT_E) <= FFT_THREADS, "2n-point transform needs too many threads");
         const int tN = tmax(N, st_N), tn = tmax(n, st_n);
         RTV_REQUIRE(tN >= 1 && 8 * st_N <= lds_max, "2n-point transform does not fit in LDS");
-        T_rows = std::min(std::min(M, tN), FR_MAXT);
+        // forward rows: the basis decomposition reads all M frames per pixel, so
+        // more basis functions per workgroup (up to FFT_THREADS threads) pay off
+        const int tR = std::max(1, std::min(FFT_THREADS / fft_tpt(N, FFT_E), lds_max / (8 * st_N)));
+        T_rows = std::min(std::min(M, tR), FR_MAXT);
         T_cols = 1;
         while (T_cols * 2 <= std::min(tN, 16) && N % (T_cols * 2) == 0) T_cols *= 2;
         T_arows = std::min(tN, 16);
