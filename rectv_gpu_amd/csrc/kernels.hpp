@@ -430,6 +430,29 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
     for (int zz = 0; zz < SM_ZW; zz++)
 #pragma unroll
         for (int b = 0; b < NB; b++) acc[zz][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+    // chunk staging data are prefetched into registers one chunk ahead, so the
+    // dependent global loads (id -> geometry, q, phi) overlap the MFMA loop
+    int pid = -1;
+    SampleGeom pg = {0, 0, 0.f, 0.f};
+    float2 pq[SM_ZW], pph[(32 + 7) / 8];
+    auto prefetch = [&](int c0) {
+        const int cnt = min(SM_CH, te.count - c0);
+        pid = -1;
+        if (sm < cnt) {
+            pid = sids[te.start + c0 + sm];
+            pg = geo[pid];
+#pragma unroll
+            for (int zz = 0; zz < SM_ZW; zz++)
+                pq[zz] = zz < zc ? gq[(size_t)(z0 + zz) * nt * n + pid] : make_float2(0.f, 0.f);
+            const float2 *ph = phik + (size_t)(pid / n) * M;
+#pragma unroll
+            for (int q = 0; q < (32 + 7) / 8; q++) {
+                const int j = part + 8 * q;
+                pph[q] = j < M ? ph[j] : make_float2(0.f, 0.f);
+            }
+        }
+    };
+    if (te.count > 0) prefetch(0);
     for (int c0 = 0; c0 < te.count; c0 += SM_CH) {
         const int cnt = min(SM_CH, te.count - c0);
         __syncthreads();
@@ -437,9 +460,8 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
             bfirst[tid] = cnt;
             blast[tid] = 0;
         }
-        if (sm < cnt) {
-            const int id = sids[te.start + c0 + sm];
-            const SampleGeom g = geo[id];
+        if (pid >= 0) {
+            const SampleGeom g = pg;
             int dy = g.row0 - ty0;
             if (dy < 0) dy += N;
             int dx = g.col0 - tx0;
@@ -456,18 +478,24 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
                 wyr[sm][i] = ((unsigned)ay < (unsigned)KW) ? expf(-beta * dyv * dyv) : 0.f;
                 wxr[sm][i] = ((unsigned)ax < (unsigned)KW) ? expf(-beta * dxv * dxv) : 0.f;
             }
-            // P columns for j = 2*part, 2*part+1 (and the zero padding)
-            const float2 *ph = phik + (size_t)(id / n) * M;
-            for (int zz = 0; zz < zc; zz++) {
-                const float2 qv = gq[(size_t)(z0 + zz) * nt * n + id];
-                for (int j = part; j < M; j += 8) {
-                    const float2 p = cmulc(qv, ph[j]);  // q * conj(phi_j[k])
-                    sP[zz][sm][j] = p.x;
-                    sP[zz][sm][M + j] = p.y;
+            // P columns for j = part, part + 8, ... (and the zero padding)
+#pragma unroll
+            for (int zz = 0; zz < SM_ZW; zz++) {
+                if (zz < zc) {
+#pragma unroll
+                    for (int q = 0; q < (32 + 7) / 8; q++) {
+                        const int j = part + 8 * q;
+                        if (j < M) {
+                            const float2 p = cmulc(pq[zz], pph[q]);  // q * conj(phi_j[k])
+                            sP[zz][sm][j] = p.x;
+                            sP[zz][sm][M + j] = p.y;
+                        }
+                    }
+                    for (int c = 2 * M + part; c < NC; c += 8) sP[zz][sm][c] = 0.f;
                 }
-                for (int c = 2 * M + part; c < NC; c += 8) sP[zz][sm][c] = 0.f;
             }
         }
+        if (c0 + SM_CH < te.count) prefetch(c0 + SM_CH);
         __syncthreads();
         // first/last index of each window-row bin (the list is sorted by ly0)
         if (tid < cnt) {
