@@ -55,6 +55,35 @@ def byte_model(n, nt, m, nz, nzp, titer):
     return titer * inner + 12 * V + 48 * V4
 
 
+def copy_probe(nbytes=1 << 30):
+    """A device-to-device copy of known size (for calibrating rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters against a plain copy kernel)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def measured_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/traffic_<config>.json, FETCH_SIZE + WRITE_SIZE calibrated on a copy
+    of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % config)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("bytes_per_launch")
+
+
 def cpu_baseline(w, titer):
     """FP32 NumPy port of the same solver (the oracle in float32) on a bounded sample:
     one slice of the workload (all projections, all basis functions), one inner step
@@ -93,6 +122,7 @@ def main():
     ap.add_argument("--config", default="full")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true", help="also run a 1 GiB device copy (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build drives exactly one GPU; --gpus %d is not supported "
@@ -145,7 +175,8 @@ def main():
     davg = prof[dom][1] / prof[dom][0] * 1e-3
     achieved = db / davg / 1e9
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4), "traffic": None}
+            "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4),
+            "traffic": measured_traffic(a.config, dom), "alg_bytes_per_launch": db}
     B = byte_model(n, nt, m, nz, nzp, a.titer)
     s.free()
     out = {
@@ -160,6 +191,8 @@ def main():
         "roofline": roof,
         "kernels": kern,
     }
+    if a.copy_probe:
+        copy_probe()
     if not a.no_cpu:
         print("bench: CPU baseline sample ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, a.titer)
