@@ -24,21 +24,22 @@ HBM_PEAK = 8.0e12
 
 def kernel_bytes(name, n, nt, m, zc, nzp):
     """Algorithmic HBM bytes of one launch of each kernel over zc slices (each input
-    read once, each output written once)."""
+    read once, each output written once).  The M real frames of a slice are
+    transformed as ceil(M/2) complex pairs, so there are S = zc*ceil(M/2) spectra."""
     N = 2 * n
     V = zc * m * n * n                       # voxels
     V4 = (zc // nzp) * (nzp + 1) * (m + 1) * (n + 1) ** 2  # dual entries
-    S = zc * m                               # spectra
+    S = zc * ((m + 1) // 2)                  # pair spectra
     Gs = zc * nt * n                         # polar samples
     return {
         "fwd_rows": 4 * V + S * n * N * 8,
         "fwd_cols": S * (n * N + N * N) * 8,
-        "gather": S * N * N * 8 + Gs * 8,
-        "ray": Gs * (8 + 4 + 8),
+        "gather": S * N * N * 8 + Gs * 16,
+        "ray": Gs * (16 + 4 + 8),
         "scatter": Gs * 8 + S * N * N * 8,
         "adj_cols": S * (N * N + n * N) * 8,
         "adj_rows": S * (n * N + n * n) * 8,
-        "update": V * (4 + 4 + 8) + 32 * V4,
+        "update": V * (4 + 4 + 4) + 32 * V4,
         "prox": 8 * V + 48 * V4,
     }.get(name)
 

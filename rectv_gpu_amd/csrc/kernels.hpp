@@ -1,18 +1,29 @@
 // HIP kernels of the rectv reconstruction path (gfx950).
 //
+// The M time frames of a slice are real images, so they are transformed in
+// pairs: Z_p = FFT2(f_{2p} + i f_{2p+1}) holds both frame spectra (F_a(c) =
+// (Z(c) + conj Z(-c))/2, F_b(c) = (Z(c) - conj Z(-c))/(2i)).  The basis
+// contraction sum_j phi_j[k] conj(phi_j[t_i]) = D[k][i] becomes, per pair, one
+// coefficient on Z at the sample's own 9x9 window ("direct" item) and one on Z
+// at the point-mirrored window ("mirror" item, conjugated); the adjoint is the
+// transpose and yields Hermitian-symmetric grids whose inverse transforms are
+// the frame pairs themselves.  MP = ceil(M/2) spectra per slice.
+//
 // Data layouts in HBM (z = slice within the current slice batch):
 //   f        [nz][M][n][n]                 f32   time frames of each slice
-//   H        [zb][M][n][2n]                c64   row-transformed spectra (forward),
+//   H        [zb][MP][n][2n]               c64   row-transformed spectra (forward),
 //                                                 column-transformed (adjoint)
-//   G        [zb][M][2n][2n]               c64   oversampled spectra, centred
-//   gq       [zb][ntheta][n]               c64   polar samples (gather out / scatter in)
-//   v        [zb][M][n][n]                 c64   adjoint images per basis function
+//   G        [zb][MP][2n][2n]              c64   oversampled spectra, centred
+//   gq       [zb][ntheta][n][2]            c64   per sample: direct / mirror gather
+//                                                 sums (forward); slot 0 = q (adjoint)
+//   v        [zb][MP][n][n]                c64   adjoint frame pairs (re = f_{2p}, im = f_{2p+1})
 //   psi, mu  [nchunk][nzp+1][M+1][n+1][n+1] f32x4 dual variables (x, y, t, z)
 // The oversampled grid index kb in [0, 2n) carries the centred frequency
-// kb - n; the checkerboard (-1)^(a+b) before and after the 2D FFT (the
-// reference's fftshift on the (2n+2K)^2 grid, kernels_radonusfft.cuh:26-38) is
-// applied separably in the row and column passes, and the periodic border of
-// that grid (wrap, kernels_radonusfft.cuh:40-64) becomes index arithmetic mod 2n.
+// kb - n (the mirror of kb is (2n - kb) mod 2n); the checkerboard (-1)^(a+b)
+// before and after the 2D FFT (the reference's fftshift on the (2n+2K)^2 grid,
+// kernels_radonusfft.cuh:26-38) is applied separably in the row and column
+// passes, and the periodic border of that grid (wrap, kernels_radonusfft.cuh:40-64)
+// becomes index arithmetic mod 2n.
 #pragma once
 #include "common.hpp"
 #include "fft_lds.hpp"
@@ -25,7 +36,6 @@ constexpr int TS = 16;          // spectral tile edge (cells) for gather/scatter
 constexpr int KW = 9;           // Gaussian window taps per axis, 2K+1 with K = 4
 constexpr int TH = TS + KW - 1; // tile + halo rows loaded by the gather
 constexpr int TP = TH + 1;      // LDS row pitch of the gather tile
-constexpr int FR_MAXT = 16;     // max basis functions per forward-row workgroup
 
 __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 
@@ -53,61 +63,45 @@ struct TileEntry {
 };
 
 // ---------------------------------------------------------------------------
-// F1: basis decomposition + deapodisation + zero-padded row FFT.
-// Replaces makecomplex/decphi (kernels_operators.cuh:86-116) and divphi FWD +
-// the first half of the 2D cuFFT (kernels_radonusfft.cuh:4-24, radonusfft.cu:85-89).
-// One workgroup per (slice z, image row y, group of T basis functions).
+// F1: frame pairs f_{2p} + i f_{2p+1} + deapodisation + zero-padded row FFT
+// (makecomplex/decphi, kernels_operators.cuh:86-116, is replaced by the pair
+// packing; divphi FWD + the first half of the 2D cuFFT, kernels_radonusfft.cuh:4-24,
+// radonusfft.cu:85-89).  One workgroup per (slice z, image row y, T pairs).
 template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float2 *__restrict__ phit,
-           const float *__restrict__ deap, FftPlan plan, const float2 *__restrict__ tw, int n, int M, int T,
-           int stride) {
+k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__restrict__ deap, FftPlan plan,
+           const float2 *__restrict__ tw, int n, int M, int MP, int T, int stride) {
     extern __shared__ float2 lds[];
     if constexpr (NP > 0) n = NP / 2;
+    if constexpr (NP > 0) T = fft_rows_p2(NP);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int ngr = (M + T - 1) / T;
-    const int jg = L % ngr;
+    const int ngr = (MP + T - 1) / T;
+    const int pg = L % ngr;
     const int y = (L / ngr) % n;
     const int z = L / ngr / n;
-    const int j0 = jg * T;
-    const int Tc = min(T, M - j0);
+    const int p0 = pg * T;
+    const int Tc = min(T, MP - p0);
     const int N = NP ? NP : 2 * n, h = N / 4;
     const size_t plane = (size_t)n * n;
     const float *fz = f + (size_t)z * M * plane + (size_t)y * n;
     const float dy = deap[y];
-    // each thread reads the M frames of its pixels once and forms all Tc u_j
-    for (int x = tid; x < n; x += nthr) {
-        float2 acc[FR_MAXT];
-#pragma unroll
-        for (int jt = 0; jt < FR_MAXT; jt++) acc[jt] = make_float2(0.f, 0.f);
-        for (int i = 0; i < M; i++) {
-            const float v = fz[(size_t)i * plane + x];
-#pragma unroll
-            for (int jt = 0; jt < FR_MAXT; jt++) {
-                if (jt < Tc) {
-                    const float2 c = phit[(size_t)(j0 + jt) * M + i];  // phi_j[t_i]; use its conjugate
-                    acc[jt].x += v * c.x;
-                    acc[jt].y -= v * c.y;
-                }
-            }
-        }
-        const float s = deap[x] * dy * sgn(x + h);
+    for (int idx = tid; idx < Tc * n; idx += nthr) {
+        const int pt = idx / n, x = idx - pt * n;
+        const int i0 = 2 * (p0 + pt);
+        const float va = fz[(size_t)i0 * plane + x];
+        const float vb = i0 + 1 < M ? fz[(size_t)(i0 + 1) * plane + x] : 0.f;
+        const float sc = deap[x] * dy * sgn(x + h);
+        lds[pt * stride + lp(x + h)] = make_float2(va * sc, vb * sc);
         const int q = x < h ? x : x + n;  // zero padding
-#pragma unroll
-        for (int jt = 0; jt < FR_MAXT; jt++) {
-            if (jt < Tc) {
-                lds[jt * stride + lp(x + h)] = cscale(acc[jt], s);
-                lds[jt * stride + lp(q)] = make_float2(0.f, 0.f);
-            }
-        }
+        lds[pt * stride + lp(q)] = make_float2(0.f, 0.f);
     }
     __syncthreads();
     fft_any<NP, FFT_E, false>(lds, stride, Tc, plan, tw, tid, nthr);
-    float2 *Hz = H + ((size_t)z * M + j0) * n * N + (size_t)y * N;
+    float2 *Hz = H + ((size_t)z * MP + p0) * n * N + (size_t)y * N;
     for (int idx = tid; idx < Tc * N; idx += nthr) {
-        const int jt = idx / N, kx = idx - jt * N;
-        Hz[(size_t)jt * n * N + kx] = cscale(lds[jt * stride + lp(kx)], sgn(kx));
+        const int pt = idx / N, kx = idx - pt * N;
+        Hz[(size_t)pt * n * N + kx] = cscale(lds[pt * stride + lp(kx)], sgn(kx));
     }
 }
 
@@ -155,23 +149,27 @@ __device__ __forceinline__ void window_weights(float frac, float beta, float *w)
     }
 }
 
-// F3: tile-binned Gaussian gather from the M spectra of slice z, contracted with
-// phi_j[k] (replaces gather FWD kernels_radonusfft.cuh:81-144 run once per basis
-// function, plus mulphi/addreal's sum over j, operators.cu:22-31).
-// Work unit: a group of up to GSZ samples sharing one 9x9 window origin (rays
+// F3: tile-binned Gaussian gather from the MP pair spectra of slice z (replaces
+// gather FWD kernels_radonusfft.cuh:81-144 run once per basis function, plus
+// mulphi/addreal's sum over j, operators.cu:22-31).
+// Items: sample (k, s) has a direct item 2*(k*n+s) (its own window) and a mirror
+// item 2*(k*n+s)+1 (the point-mirrored window, fractions negated).  Item sums
+// are contracted with cf[k][kind][p] (alpha_p[k] direct, conj(beta_p[k]) mirror)
+// and written to gq[z][item]; the per-ray kernel forms direct + conj(mirror).
+// Work unit: a group of up to GSZ items sharing one 9x9 window origin (rays
 // repeated every 2 pi, and rays theta+pi read in reverse detector order, land on
 // the same frequency cells).  One workgroup per (tile entry of <= GMAX groups,
 // slice z).  The group weights are staged once in LDS; the spectra tile plus its
-// 8-cell halo is staged JC basis functions at a time; lane = (group, j) with j
-// fastest, so each lane reads the window taps of one spectrum once for all group
-// members, and the contraction over j is a JC-lane shuffle reduction.
+// 8-cell halo is staged GJC spectra at a time; lane = (group, p) with p fastest,
+// so each lane reads the window taps of one spectrum once for all group
+// members, and the contraction over p is a GJC-lane shuffle reduction.
 constexpr int GMAX = 64;     // groups per gather workgroup
 constexpr int GJC = 4;       // spectra staged per pass (power of two)
 template <int GSZ>
 __global__ void __launch_bounds__(256)
 k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
-         const int4 *__restrict__ groups, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik,
-         int n, int nt, int M, float beta, int nentries) {
+         const int4 *__restrict__ groups, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf,
+         int n, int nt, int MP, float beta, int nentries) {
     __shared__ float2 tile[GJC][TH][TP];
     __shared__ float wxs[GMAX][GSZ][KW];  // column weights per member
     __shared__ float wys[GMAX][KW][GSZ];  // row weights, members contiguous per row
@@ -186,7 +184,7 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
     const TileEntry te = tiles[e];
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
     const int cnt = te.count;
-    const float2 *Gz = G + (size_t)z * M * N * N;
+    const float2 *Gz = G + (size_t)z * MP * N * N;
     // stage group geometry and separable weights
     for (int t = tid; t < cnt * GSZ; t += nthr) {
         const int g = t / GSZ, q = t - g * GSZ;
@@ -215,8 +213,8 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
             }
         }
     }
-    for (int jc = 0; jc < M; jc += GJC) {
-        const int Jc = min(GJC, M - jc);
+    for (int jc = 0; jc < MP; jc += GJC) {
+        const int Jc = min(GJC, MP - jc);
         // stage the tile: every global load of a thread is issued before its LDS
         // stores (fixed trip count, unrolled) so the memory latency is paid once
         constexpr int NLD = (GJC * TH * TH + 255) / 256;
@@ -287,7 +285,7 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
 #pragma unroll
                 for (int q = 0; q < GSZ; q++) {
                     const int id = gid[g][q];
-                    if (id >= 0) c[q] = cmul(phik[(size_t)(id / n) * M + jc + jl], sum[q]);
+                    if (id >= 0) c[q] = cmul(cf[((size_t)((id >> 1) / n) * 2 + (id & 1)) * MP + jc + jl], sum[q]);
                 }
             }
             // sum the GJC basis functions of each group (lanes g*GJC .. g*GJC+GJC-1)
@@ -305,7 +303,7 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
         }
     }
     __syncthreads();
-    float2 *out = gq + (size_t)z * nt * n;
+    float2 *out = gq + (size_t)z * nt * n * 2;
     for (int t = tid; t < cnt * GSZ; t += nthr) {
         const int g = t / GSZ, q = t - g * GSZ;
         const int id = gid[g][q];
@@ -314,11 +312,13 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
 }
 
 // F4: per-ray 1D transforms (one workgroup per T rays).
-//   FWD: h1[s'] = Re{post_f[s'] * IFFT(pre_f[s] * gq[s])} (- g)   (shift, fftshift,
+//   FWD: h1[s'] = Re{post_f[s'] * IFFT(pre_f[s] * (P[s] + conj Q[s]))} (- g), with P, Q
+//        the direct and mirror gather sums of sample s   (shift, fftshift,
 //        CUFFT_INVERSE, fftshift, mulr, mulphi scale: radonusfft.cu:96-102,
 //        kernels_operators.cuh:135-149), optional residual norm partials.
 //   ADJ: q[s] = post_a[s] * FFT(pre_a[s'] * h1[s'])  (radonusfft.cu:115-121 and
-//        the 1/n, s == 0 rule of the adjoint gather, kernels_radonusfft.cuh:102-108).
+//        the 1/n, s == 0 rule of the adjoint gather, kernels_radonusfft.cuh:102-108),
+//        written to slot 0 of sample s.
 enum { RAY_FWD = 1, RAY_SUBG = 2, RAY_ADJ = 4 };
 template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
@@ -339,7 +339,10 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
             float2 v = make_float2(0.f, 0.f);
-            if (s != 0) v = cmul(gq[(size_t)(r0 + t) * n + s], pre_f[s]);
+            if (s != 0) {
+                const float4 pq = reinterpret_cast<const float4 *>(gq)[(size_t)(r0 + t) * n + s];
+                v = cmul(make_float2(pq.x + pq.z, pq.y - pq.w), pre_f[s]);
+            }
             lds[t * stride + lp(s)] = v;
         }
         __syncthreads();
@@ -366,7 +369,7 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
             const int t = idx / n, s = idx - t * n;
             float2 v = make_float2(0.f, 0.f);
             if (s != 0) v = cmul(lds[t * stride + lp(s)], post_a[s]);
-            gq[(size_t)(r0 + t) * n + s] = v;
+            gq[((size_t)(r0 + t) * n + s) * 2] = v;
         }
     }
     if (partial) {
@@ -389,22 +392,25 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // with global atomicAdd, kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62,
 // run once per basis function after mulphi(-1), operators.cu:47-56).
 // One workgroup (16 waves) owns one TS x TS tile of the 2n grid for ZW slices;
-// wave r owns tile row r.  The samples whose 9x9 window intersects the tile are
-// listed sorted by window row.  A chunk of them is staged in LDS once for the ZW
-// slices: 16-wide weight rows wyr[s][r] = wy[r-ly0], wxr[s][i] = wx[i-lx0]
-// (zero outside the window) and, per slice, P[s][col] = q*conj(phi_j[k]) split
-// into real (col = j) and imaginary (col = M + j) columns.  For its row the wave
-// takes the run of samples covering it and accumulates
+// wave r owns tile row r.  The items (direct and mirror windows of the samples,
+// see F3) whose 9x9 window intersects the tile are listed sorted by window row.
+// A chunk of them is staged in LDS once for the ZW slices: 16-wide weight rows
+// wyr[s][r] = wy[r-ly0], wxr[s][i] = wx[i-lx0] (zero outside the window) and,
+// per slice, P[s][col] = conj(cf[k][kind][p]) * q' (q' = q direct, conj(q)
+// mirror) split into real (col = p) and imaginary (col = MP + p) columns.  The
+// result is the Hermitian-symmetrised pair grid whose inverse transform is
+// f_{2p} + i f_{2p+1}.  For its row the wave takes the run of items covering it
+// and accumulates
 //   D[cell i][col] += sum_k wyr[k][r] * wxr[k][i] * P[k][col]
-// with v_mfma_f32_16x16x4f32 (16 cells x 16 columns x 4 samples per
+// with v_mfma_f32_16x16x4f32 (16 cells x 16 columns x 4 items per
 // instruction).  No atomics; fixed summation order.
 constexpr int SM_CH = 128;  // samples staged per chunk (8 staging threads each)
 constexpr int SM_ZW = 2;    // slices per workgroup
-template <int NB>           // 16-column blocks: 2*M <= 16*NB
+template <int NB>           // 16-column blocks: 2*MP <= 16*NB
 __global__ void __launch_bounds__(1024)
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
-          const int *__restrict__ sids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ phik, int n,
-          int nt, int M, float beta, int ntiles, int nzb) {
+          const int *__restrict__ sids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf, int n,
+          int nt, int MP, float beta, int ntiles, int nzb) {
     constexpr int NC = 16 * NB;
     constexpr int NCP = NC + 1;  // padded row of P
     constexpr int NBIN = TS + KW - 1;  // window rows ly0 in [-8, TS)
@@ -431,24 +437,29 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
 #pragma unroll
         for (int b = 0; b < NB; b++) acc[zz][b] = (v4f){0.f, 0.f, 0.f, 0.f};
     // chunk staging data are prefetched into registers one chunk ahead, so the
-    // dependent global loads (id -> geometry, q, phi) overlap the MFMA loop
+    // dependent global loads (item -> geometry, q, coefficients) overlap the MFMA loop
     int pid = -1;
     SampleGeom pg = {0, 0, 0.f, 0.f};
-    float2 pq[SM_ZW], pph[(32 + 7) / 8];
+    float2 pq[SM_ZW], pph[(NC / 2 + 7) / 8];
     auto prefetch = [&](int c0) {
         const int cnt = min(SM_CH, te.count - c0);
         pid = -1;
         if (sm < cnt) {
             pid = sids[te.start + c0 + sm];
             pg = geo[pid];
+            const int sid = pid >> 1;
+            const bool mir = pid & 1;
 #pragma unroll
-            for (int zz = 0; zz < SM_ZW; zz++)
-                pq[zz] = zz < zc ? gq[(size_t)(z0 + zz) * nt * n + pid] : make_float2(0.f, 0.f);
-            const float2 *ph = phik + (size_t)(pid / n) * M;
+            for (int zz = 0; zz < SM_ZW; zz++) {
+                float2 q = zz < zc ? gq[((size_t)(z0 + zz) * nt * n + sid) * 2] : make_float2(0.f, 0.f);
+                if (mir) q.y = -q.y;
+                pq[zz] = q;
+            }
+            const float2 *ph = cf + ((size_t)(sid / n) * 2 + (pid & 1)) * MP;
 #pragma unroll
-            for (int q = 0; q < (32 + 7) / 8; q++) {
+            for (int q = 0; q < (NC / 2 + 7) / 8; q++) {
                 const int j = part + 8 * q;
-                pph[q] = j < M ? ph[j] : make_float2(0.f, 0.f);
+                pph[q] = j < MP ? ph[j] : make_float2(0.f, 0.f);
             }
         }
     };
@@ -478,20 +489,20 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
                 wyr[sm][i] = ((unsigned)ay < (unsigned)KW) ? expf(-beta * dyv * dyv) : 0.f;
                 wxr[sm][i] = ((unsigned)ax < (unsigned)KW) ? expf(-beta * dxv * dxv) : 0.f;
             }
-            // P columns for j = part, part + 8, ... (and the zero padding)
+            // P columns for p = part, part + 8, ... (and the zero padding)
 #pragma unroll
             for (int zz = 0; zz < SM_ZW; zz++) {
                 if (zz < zc) {
 #pragma unroll
-                    for (int q = 0; q < (32 + 7) / 8; q++) {
+                    for (int q = 0; q < (NC / 2 + 7) / 8; q++) {
                         const int j = part + 8 * q;
-                        if (j < M) {
-                            const float2 p = cmulc(pq[zz], pph[q]);  // q * conj(phi_j[k])
+                        if (j < MP) {
+                            const float2 p = cmulc(pq[zz], pph[q]);  // q' * conj(cf)
                             sP[zz][sm][j] = p.x;
-                            sP[zz][sm][M + j] = p.y;
+                            sP[zz][sm][MP + j] = p.y;
                         }
                     }
-                    for (int c = 2 * M + part; c < NC; c += 8) sP[zz][sm][c] = 0.f;
+                    for (int c = 2 * MP + part; c < NC; c += 8) sP[zz][sm][c] = 0.f;
                 }
             }
         }
@@ -536,10 +547,10 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
 #pragma unroll
             for (int v = 0; v < 4; v++) wl[(4 * lk + v) * NC + 16 * b + li] = acc[zz][b][v];
         __syncthreads();
-        float2 *Gr = Gt + (size_t)(z0 + zz) * M * N * N;
-        for (int idx = l; idx < M * 16; idx += 64) {
+        float2 *Gr = Gt + (size_t)(z0 + zz) * MP * N * N;
+        for (int idx = l; idx < MP * 16; idx += 64) {
             const int j = idx >> 4, i = idx & 15;
-            Gr[(size_t)j * N * N + i] = make_float2(wl[i * NC + j], wl[i * NC + M + j]);
+            Gr[(size_t)j * N * N + i] = make_float2(wl[i * NC + j], wl[i * NC + MP + j]);
         }
     }
 }
@@ -606,24 +617,20 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
     }
 }
 
-// f[z][i] (+)= tau * Re sum_j v_j phi_j[t_i]   (recphi + addreal,
-// kernels_operators.cuh:118-133 and :75-84).  Used by the standalone R^* entry point.
-__global__ void k_recphi(float *__restrict__ f, const float2 *__restrict__ v, const float2 *__restrict__ phit,
-                         int n, int M, float tau, int accumulate) {
+// f[z][i] (+)= tau * frame i of the adjoint pairs v (re of pair i/2 for even i,
+// im for odd i): recphi + addreal, kernels_operators.cuh:118-133 and :75-84.
+// Used by the standalone R^* entry point.
+__global__ void k_unpack(float *__restrict__ f, const float2 *__restrict__ v, int n, int M, float tau, int accumulate) {
     const size_t plane = (size_t)n * n;
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
     if (p >= plane) return;
-    const float2 *vz = v + (size_t)z * M * plane + p;
+    const int MP = (M + 1) / 2;
+    const float2 *vz = v + (size_t)z * MP * plane + p;
     float *fz = f + (size_t)z * M * plane + p;
     for (int i = 0; i < M; i++) {
-        float s = 0.f;
-        for (int j = 0; j < M; j++) {
-            const float2 a = vz[(size_t)j * plane];
-            const float2 c = phit[(size_t)j * M + i];
-            s += a.x * c.x - a.y * c.y;
-        }
-        const float val = tau * s;
+        const float2 a = vz[(size_t)(i >> 1) * plane];
+        const float val = tau * ((i & 1) ? a.y : a.x);
         fz[(size_t)i * plane] = accumulate ? fz[(size_t)i * plane] + val : val;
     }
 }
@@ -644,7 +651,8 @@ __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i
 }
 
 // U: one inner ADMM step of the TV part fused with the R^* update (admm.cu:85-97):
-//   out = fm - 0.5*div(rho*(grad fm - psi) + mu) - 0.5*Re sum_j v_j phi_j[t_i]
+//   out = fm - 0.5*div(rho*(grad fm - psi) + mu) - 0.5*(R^* h1)_i
+// with (R^* h1)_i = Re sum_j v_j phi_j[t_i] read directly from the frame pairs v.
 // Thread per (chunk, y, x) column marching over the chunk's slices and frames.
 // Halo slabs (chunk neighbours) come from A, the previous outer iterate
 // (rectv.cu:170-171; clamped borders kernels_operators.cuh:4-20).  The dual
@@ -654,7 +662,7 @@ template <int MM>
 __global__ void __launch_bounds__(256)
 k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__restrict__ out,
          const float4 *__restrict__ psi, const float4 *__restrict__ mu, const float2 *__restrict__ v,
-         const float2 *__restrict__ phit, TvArgs a, int zv0) {
+         TvArgs a, int zv0) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int c = a.chunk0 + blockIdx.z;
@@ -687,17 +695,18 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
                 else fzp[i] = flgr ? A[(zg + 1) * S + i * plane + col] : fz[i];
             }
         }
-        // R^* part: rp_i = Re sum_j v_j phi_j[t_i]
+        // R^* part: frame i of the adjoint pairs
 #pragma unroll
         for (int i = 0; i < MM; i++) rp[i] = 0.f;
         if (v) {
-            const float2 *vz = v + ((size_t)(zg - zv0) * M) * plane + col;
-            for (int j = 0; j < M; j++) {
-                const float2 vj = vz[(size_t)j * plane];
-                const float2 *cj = phit + (size_t)j * M;
+            const float2 *vz = v + ((size_t)(zg - zv0) * ((M + 1) / 2)) * plane + col;
 #pragma unroll
-                for (int i = 0; i < MM; i++)
-                    if (i < M) rp[i] += vj.x * cj[i].x - vj.y * cj[i].y;
+            for (int pp = 0; pp < MM / 2; pp++) {
+                if (2 * pp < M) {
+                    const float2 vp = vz[(size_t)pp * plane];
+                    rp[2 * pp] = vp.x;
+                    rp[2 * pp + 1] = vp.y;
+                }
             }
         }
         // t-gradient dual value on frame plane 0: E(t'=1) - E(t'=0) = 0

@@ -63,10 +63,10 @@ struct HostGeom {
     std::vector<int4> fgroups;
     std::vector<int> asids;
     std::vector<TileEntry> ftiles, atiles;
-    std::vector<float2> pre_f, post_a, phik, phit;
+    std::vector<float2> pre_f, post_a, cf;
     std::vector<float> post_f, pre_a, deap;
     float beta = 0.f;
-    int fgs = 4;  // samples per gather group
+    int fgs = 4;  // items per gather group
 };
 
 // USFFT constants, radonusfft.cu:10-12: mu = -log(eps)/(2 n^2) held in float.
@@ -87,8 +87,9 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
     const double mud = mu;
     hg.beta = (float)(PI_REF * PI_REF / (mud * 4.0 * n * n));
     const size_t ns = (size_t)nt * n;
-    hg.geo.assign(ns, SampleGeom{0, 0, 0.f, 0.f});
-    std::vector<int> ftile(ns, -1);
+    const size_t ni = 2 * ns;  // items: 2*id direct window, 2*id+1 point-mirrored window
+    hg.geo.assign(ni, SampleGeom{0, 0, 0.f, 0.f});
+    std::vector<int> ftile(ni, -1);
     // polar sample positions (takexy, kernels_radonusfft.cuh:66-79) in FP64
     for (int k = 0; k < nt; k++) {
         const double th = (double)theta[k];
@@ -105,12 +106,20 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             g.row0 = (int)(((long long)n + (long long)fyl - K) % N + N) % N;
             g.fx = (float)(X - fxl);
             g.fy = (float)(Y - fyl);
+            // mirror: cells (2n - c) mod 2n of the window, taps reversed, fractions negated
+            SampleGeom m;
+            m.row0 = (2 * N - g.row0 - (KW - 1)) % N;
+            m.col0 = (2 * N - g.col0 - (KW - 1)) % N;
+            m.fy = -g.fy;
+            m.fx = -g.fx;
             const size_t id = (size_t)k * n + s;
-            hg.geo[id] = g;
-            ftile[id] = (g.row0 / TS) * nT + g.col0 / TS;
+            hg.geo[2 * id] = g;
+            hg.geo[2 * id + 1] = m;
+            ftile[2 * id] = (g.row0 / TS) * nT + g.col0 / TS;
+            ftile[2 * id + 1] = (m.row0 / TS) * nT + m.col0 / TS;
         }
     }
-    // Work lists.  Samples are grouped by window origin (up to GS per group, ids
+    // Work lists.  Items are grouped by window origin (up to GS per group, ids
     // ascending) within each tile; forward: the tile holding the origin (split
     // into entries of at most chunk_max groups); adjoint: every tile the 9x9
     // window intersects (one entry per tile, empty tiles included).
@@ -155,8 +164,8 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
     };
     {
         std::vector<Key> keys;
-        keys.reserve(ns);
-        for (size_t id = 0; id < ns; id++) {
+        keys.reserve(ni);
+        for (size_t id = 0; id < ni; id++) {
             if (ftile[id] < 0) continue;
             const SampleGeom &g = hg.geo[id];
             keys.push_back(Key{((unsigned long long)ftile[id] << 32) | (unsigned)(g.row0 * N + g.col0), (int)id});
@@ -171,12 +180,12 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         hg.fgs = share >= 3.0 ? 4 : (share >= 1.6 ? 2 : 1);
         make_groups(keys, hg.fgroups, hg.ftiles, false, chunk_max, hg.fgs);
     }
-    // adjoint lists: every tile the 9x9 window intersects, each tile's samples
+    // adjoint lists: every tile the 9x9 window intersects, each tile's items
     // sorted by (local window row, local window column, id)
     {
         std::vector<Key> keys;
-        keys.reserve(3 * ns);
-        for (size_t id = 0; id < ns; id++) {
+        keys.reserve(3 * ni);
+        for (size_t id = 0; id < ni; id++) {
             if (ftile[id] < 0) continue;
             const SampleGeom &g = hg.geo[id];
             const int ty[2] = {g.row0 / TS, ((g.row0 + KW - 1) % N) / TS};
@@ -228,14 +237,35 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         const double t = x - n / 2;
         hg.deap[x] = (float)(1.0 / (2.0 * n * std::exp(-mud * t * t)));
     }
-    // basis tables: phik[k][j] = phi_j[k], phit[j][i] = phi_j[t_i], t_i = i*nt/M (int)
-    hg.phik.resize((size_t)nt * M);
+    // pair coefficients.  D[k][i] = sum_j phi_j[k] conj(phi_j[t_i]), t_i = i*nt/M (int)
+    // (mulphi with decphi / recphi, kernels_operators.cuh:99-149); pair p = (a, b) =
+    // (2p, 2p+1): cf[k][0][p] = (D_a - i D_b)/2, cf[k][1][p] = conj((D_a + i D_b)/2).
+    const int MP = (M + 1) / 2;
+    hg.cf.assign((size_t)nt * 2 * MP, make_float2(0.f, 0.f));
+    std::vector<double> pr((size_t)M * nt), pim((size_t)M * nt);
     for (int j = 0; j < M; j++)
-        for (int k = 0; k < nt; k++)
-            hg.phik[(size_t)k * M + j] = make_float2(phi[(size_t)j * 2 * nt + 2 * k], phi[(size_t)j * 2 * nt + 2 * k + 1]);
-    hg.phit.resize((size_t)M * M);
-    for (int j = 0; j < M; j++)
-        for (int i = 0; i < M; i++) hg.phit[(size_t)j * M + i] = hg.phik[(size_t)(i * nt / M) * M + j];
+        for (int k = 0; k < nt; k++) {
+            pr[(size_t)j * nt + k] = phi[(size_t)j * 2 * nt + 2 * k];
+            pim[(size_t)j * nt + k] = phi[(size_t)j * 2 * nt + 2 * k + 1];
+        }
+    for (int k = 0; k < nt; k++) {
+        std::vector<double> dr(2 * MP, 0.0), di(2 * MP, 0.0);
+        for (int i = 0; i < M; i++) {
+            const int ti = (int)((long long)i * nt / M);
+            for (int j = 0; j < M; j++) {
+                const double ar = pr[(size_t)j * nt + k], ai = pim[(size_t)j * nt + k];
+                const double br = pr[(size_t)j * nt + ti], bi = -pim[(size_t)j * nt + ti];
+                dr[i] += ar * br - ai * bi;
+                di[i] += ar * bi + ai * br;
+            }
+        }
+        for (int p = 0; p < MP; p++) {
+            const double ar = dr[2 * p], ai = di[2 * p], br = dr[2 * p + 1], bi = di[2 * p + 1];
+            // alpha = (D_a - i D_b)/2, beta = (D_a + i D_b)/2
+            hg.cf[((size_t)k * 2 + 0) * MP + p] = make_float2((float)(0.5 * (ar + bi)), (float)(0.5 * (ai - br)));
+            hg.cf[((size_t)k * 2 + 1) * MP + p] = make_float2((float)(0.5 * (ar - bi)), (float)(-0.5 * (ai + br)));
+        }
+    }
 }
 
 // ---------------------------------------------------------------- device buffers
@@ -272,6 +302,7 @@ static void upload(DBuf<T> &b, const std::vector<T> &v, size_t &total, hipStream
 
 struct Solver {
     int n, nt, M, nz, nzp, nch, ngpus, N;
+    int MP = 1;  // frame pairs (spectra per slice)
     int dev = 0;
     hipStream_t st = nullptr;
     size_t bytes = 0;
@@ -281,7 +312,7 @@ struct Solver {
     FftPlan plan_n{}, plan_N{};
     bool odd_n = false, odd_N = false;  // lengths with factors 3, 5, 7
     DBuf<float2> d_tw;
-    int T_rows = 1, T_cols = 1, T_ray = 1, T_arows = 1, st_n = 0, st_N = 0, JC = 1;
+    int T_rows = 1, T_cols = 1, T_ray = 1, T_arows = 1, st_n = 0, st_N = 0;
     // state
     DBuf<float> d_f[3];
     int ia = 0;
@@ -299,7 +330,7 @@ struct Solver {
     DBuf<int4> d_fgroups;
     DBuf<int> d_asids;
     DBuf<TileEntry> d_ftiles, d_atiles;
-    DBuf<float2> d_phik, d_phit, d_pre_f, d_post_a;
+    DBuf<float2> d_cf, d_pre_f, d_post_a;
     DBuf<float> d_post_f, d_pre_a, d_deap;
     int nf_entries = 0, na_tiles = 0, fgs = 4;
     float beta = 0.f;
@@ -319,6 +350,7 @@ struct Solver {
         RTV_REQUIRE(n % 8 == 0, "n must be a multiple of 8 (16-cell spectral tiles of the 2n grid)");
         RTV_REQUIRE(M <= 32, "at most 32 basis functions are supported");
         nch = nz / nzp;
+        MP = (M + 1) / 2;
         ngpus = std::max(1, std::min(ngpus_, nch));
         try {
             init_device();
@@ -353,18 +385,18 @@ struct Solver {
         d_mu.alloc(V4, bytes);
         d_g.alloc((size_t)nz * nt * n, bytes);
         // scratch plan
-        const size_t per_slice = (size_t)M * n * N * 8 + (size_t)M * N * N * 8 + (size_t)M * n * n * 8 +
-                                 (size_t)nt * n * 8 + (size_t)nt * n * 4;
+        const size_t per_slice = (size_t)MP * n * N * 8 + (size_t)MP * N * N * 8 + (size_t)MP * n * n * 8 +
+                                 (size_t)nt * n * 16 + (size_t)nt * n * 4;
         size_t freeb = 0, totb = 0;
         RTV_HIP(hipMemGetInfo(&freeb, &totb));
         double budget = 0.45 * (double)freeb;
         if (const char *e = std::getenv("RECTV_SCRATCH_MB")) budget = std::atof(e) * 1048576.0;
         cb = (int)std::max(1.0, std::min((double)nch, std::floor(budget / ((double)per_slice * nzp))));
         zb = cb * nzp;
-        d_H.alloc((size_t)zb * M * n * N, bytes);
-        d_G.alloc((size_t)zb * M * N * N, bytes);
-        d_v.alloc((size_t)zb * M * n * n, bytes);
-        d_gq.alloc((size_t)zb * nt * n, bytes);
+        d_H.alloc((size_t)zb * MP * n * N, bytes);
+        d_G.alloc((size_t)zb * MP * N * N, bytes);
+        d_v.alloc((size_t)zb * MP * n * n, bytes);
+        d_gq.alloc((size_t)zb * nt * n * 2, bytes);
         d_h1.alloc((size_t)zb * nt * n, bytes);
         const size_t pb = (size_t)((n + 1 + 63) / 64) * ((n + 1 + 3) / 4) * nch;
         const size_t rb = (size_t)((zb * nt + T_ray - 1) / T_ray);
@@ -373,9 +405,9 @@ struct Solver {
         RTV_HIP(hipStreamSynchronize(st));
         if (std::getenv("RECTV_VERBOSE"))
             fprintf(stderr,
-                    "rectv: n=%d ntheta=%d m=%d nz=%d nzp=%d lds_max=%d T_rows=%d T_cols=%d T_ray=%d T_arows=%d JC=%d "
+                    "rectv: n=%d ntheta=%d m=%d nz=%d nzp=%d lds_max=%d T_rows=%d T_cols=%d T_ray=%d T_arows=%d "
                     "chunks/batch=%d device bytes=%zu\n",
-                    n, nt, M, nz, nzp, lds_max, T_rows, T_cols, T_ray, T_arows, JC, cb, bytes);
+                    n, nt, M, nz, nzp, lds_max, T_rows, T_cols, T_ray, T_arows, cb, bytes);
     }
 
     int fft_threads(int T, int len) const {
@@ -406,26 +438,19 @@ struct Solver {
         RTV_REQUIRE(fft_tpt(N, FFT_E) <= FFT_THREADS, "2n-point transform needs too many threads");
         const int tN = tmax(N, st_N), tn = tmax(n, st_n);
         RTV_REQUIRE(tN >= 1 && 8 * st_N <= lds_max, "2n-point transform does not fit in LDS");
-        // forward rows: the basis decomposition reads all M frames per pixel, so
-        // more basis functions per workgroup (up to FFT_THREADS threads) pay off
-        const int tR = std::max(1, std::min(FFT_THREADS / fft_tpt(N, FFT_E), lds_max / (8 * st_N)));
-        T_rows = std::min(std::min(M, tR), FR_MAXT);
         T_cols = 1;
         while (T_cols * 2 <= std::min(tN, 16) && N % (T_cols * 2) == 0) T_cols *= 2;
-        T_arows = std::min(tN, 16);
+        T_arows = T_rows = std::min(tN, 16);
         T_ray = std::min(tn, 16);
         // power-of-two lengths use compile-time row counts inside the kernels
         if ((N & (N - 1)) == 0 && N >= 64 && N <= 4096) {
-            T_cols = T_arows = fft_rows_p2(N);
+            T_cols = T_arows = T_rows = fft_rows_p2(N);
             RTV_REQUIRE(8 * st_N * T_cols <= lds_max, "LDS too small for the 2n-point transforms");
         }
         if ((n & (n - 1)) == 0 && n >= 64 && n <= 4096) {
             T_ray = fft_rows_p2(n);
             RTV_REQUIRE(8 * st_n * T_ray <= lds_max, "LDS too small for the n-point transforms");
         }
-        JC = std::max(1, std::min(M, lds_max / (TH * TP * 8)));
-        const int np = (M + JC - 1) / JC;
-        JC = (M + np - 1) / np;
         const int big = lds_max;
         const void *fns[] = {(const void *)pick_fwd_rows(N), (const void *)pick_fwd_cols(N),
                              (const void *)pick_adj_cols(N), (const void *)pick_adj_rows(N), (const void *)pick_ray(n)};
@@ -487,8 +512,7 @@ struct Solver {
         upload(d_ftiles, hg.ftiles, bytes, st);
         upload(d_asids, hg.asids, bytes, st);
         upload(d_atiles, hg.atiles, bytes, st);
-        upload(d_phik, hg.phik, bytes, st);
-        upload(d_phit, hg.phit, bytes, st);
+        upload(d_cf, hg.cf, bytes, st);
         upload(d_pre_f, hg.pre_f, bytes, st);
         upload(d_post_f, hg.post_f, bytes, st);
         upload(d_pre_a, hg.pre_a, bytes, st);
@@ -517,21 +541,21 @@ struct Solver {
         const size_t S = (size_t)M * n * n;
         (void)S;
         {
-            const int ngr = (M + T_rows - 1) / T_rows;
+            const int ngr = (MP + T_rows - 1) / T_rows;
             const int blocks = zc * n * ngr;
             const int thr = fft_threads(T_rows, N);
             const size_t lds = (size_t)T_rows * st_N * sizeof(float2);
             launch("fwd_rows", [&] {
-                hipLaunchKernelGGL(pick_fwd_rows(N), dim3(blocks), dim3(thr), lds, st, f, d_H.p, d_phit.p, d_deap.p, plan_N,
-                                   d_tw.p, n, M, T_rows, st_N);
+                hipLaunchKernelGGL(pick_fwd_rows(N), dim3(blocks), dim3(thr), lds, st, f, d_H.p, d_deap.p, plan_N,
+                                   d_tw.p, n, M, MP, T_rows, st_N);
             });
         }
         {
-            const int blocks = zc * M * (N / T_cols);
+            const int blocks = zc * MP * (N / T_cols);
             const int thr = fft_threads(T_cols, N);
             const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
             launch("fwd_cols", [&] {
-                hipLaunchKernelGGL(pick_fwd_cols(N), dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p, n, M,
+                hipLaunchKernelGGL(pick_fwd_cols(N), dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p, n, MP,
                                    T_cols, st_N);
             });
         }
@@ -541,13 +565,13 @@ struct Solver {
                 launch("gather", [&] {
                     if (fgs == 4)
                         hipLaunchKernelGGL(k_gather<4>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                           d_fgroups.p, d_ftiles.p, d_phik.p, n, nt, M, beta, nf_entries);
+                                           d_fgroups.p, d_ftiles.p, d_cf.p, n, nt, MP, beta, nf_entries);
                     else if (fgs == 2)
                         hipLaunchKernelGGL(k_gather<2>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                           d_fgroups.p, d_ftiles.p, d_phik.p, n, nt, M, beta, nf_entries);
+                                           d_fgroups.p, d_ftiles.p, d_cf.p, n, nt, MP, beta, nf_entries);
                     else
                         hipLaunchKernelGGL(k_gather<1>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                           d_fgroups.p, d_ftiles.p, d_phik.p, n, nt, M, beta, nf_entries);
+                                           d_fgroups.p, d_ftiles.p, d_cf.p, n, nt, MP, beta, nf_entries);
                 });
         }
         ray(zc, gdat, nullptr, h1out, mode, partial);
@@ -569,34 +593,34 @@ struct Solver {
         {
             const int blocks = na_tiles * ((zc + SM_ZW - 1) / SM_ZW);
             launch("scatter", [&] {
-                if (2 * M <= 16)
+                if (2 * MP <= 16)
                     hipLaunchKernelGGL(k_scatter<1>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
-                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles, zc);
-                else if (2 * M <= 32)
+                                       d_atiles.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
+                else if (2 * MP <= 32)
                     hipLaunchKernelGGL(k_scatter<2>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
-                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles, zc);
+                                       d_atiles.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
                 else
                     hipLaunchKernelGGL(k_scatter<4>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
-                                       d_atiles.p, d_phik.p, n, nt, M, beta, na_tiles, zc);
+                                       d_atiles.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
             });
         }
         {
-            const int blocks = zc * M * (N / T_cols);
+            const int blocks = zc * MP * (N / T_cols);
             const int thr = fft_threads(T_cols, N);
             const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
             launch("adj_cols", [&] {
-                hipLaunchKernelGGL(pick_adj_cols(N), dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p, n, M,
+                hipLaunchKernelGGL(pick_adj_cols(N), dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p, n, MP,
                                    T_cols, st_N);
             });
         }
         {
             const int nyt = (n + T_arows - 1) / T_arows;
-            const int blocks = zc * M * nyt;
+            const int blocks = zc * MP * nyt;
             const int thr = fft_threads(T_arows, N);
             const size_t lds = (size_t)T_arows * st_N * sizeof(float2);
             launch("adj_rows", [&] {
                 hipLaunchKernelGGL(pick_adj_rows(N), dim3(blocks), dim3(thr), lds, st, d_H.p, d_v.p, d_deap.p, plan_N,
-                                   d_tw.p, n, M, T_arows, st_N);
+                                   d_tw.p, n, MP, T_arows, st_N);
             });
         }
     }
@@ -622,8 +646,8 @@ struct Solver {
     void launch_update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
         dim3 grid((n + 63) / 64, (n + 3) / 4, ncb);
         launch("update", [&] {
-            hipLaunchKernelGGL(k_update<MM>, grid, dim3(256), 0, st, cur, A, nxt, d_psi.p, d_mu.p, d_v.p, d_phit.p,
-                               ta, zv0);
+            hipLaunchKernelGGL(k_update<MM>, grid, dim3(256), 0, st, cur, A, nxt, d_psi.p, d_mu.p, d_v.p, ta,
+                               zv0);
         });
     }
     void update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
@@ -727,9 +751,8 @@ struct Solver {
             ray(zc, nullptr, gdev + (size_t)z0 * nt * n, nullptr, RAY_ADJ, nullptr);
             rays_to_images(zc);
             dim3 grid((unsigned)(((size_t)n * n + 255) / 256), zc);
-            launch("recphi", [&] {
-                hipLaunchKernelGGL(k_recphi, grid, dim3(256), 0, st, fdev + z0 * S, d_v.p, d_phit.p, n, M, tau,
-                                   accumulate);
+            launch("unpack", [&] {
+                hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, st, fdev + z0 * S, d_v.p, n, M, tau, accumulate);
             });
         }
     }
@@ -849,8 +872,7 @@ struct Solver {
         d_asids.release(t);
         d_ftiles.release(t);
         d_atiles.release(t);
-        d_phik.release(t);
-        d_phit.release(t);
+        d_cf.release(t);
         d_pre_f.release(t);
         d_post_a.release(t);
         d_post_f.release(t);
