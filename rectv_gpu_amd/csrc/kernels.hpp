@@ -391,35 +391,40 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // A1: tile-owned Gaussian scatter on the FP32 matrix cores (replaces gather ADJ
 // with global atomicAdd, kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62,
 // run once per basis function after mulphi(-1), operators.cu:47-56).
-// One workgroup (16 waves) owns one TS x TS tile of the 2n grid for ZW slices;
-// wave r owns tile row r.  The items (direct and mirror windows of the samples,
-// see F3) whose 9x9 window intersects the tile are listed sorted by window row.
-// A chunk of them is staged in LDS once for the ZW slices: 16-wide weight rows
-// wyr[s][r] = wy[r-ly0], wxr[s][i] = wx[i-lx0] (zero outside the window) and,
-// per slice, P[s][col] = conj(cf[k][kind][p]) * q' (q' = q direct, conj(q)
-// mirror) split into real (col = p) and imaginary (col = MP + p) columns.  The
-// result is the Hermitian-symmetrised pair grid whose inverse transform is
-// f_{2p} + i f_{2p+1}.  For its row the wave takes the run of items covering it
-// and accumulates
+// One workgroup (16 waves) owns one TS x TS tile of the 2n grid for SM_ZW
+// slices; wave r owns tile row r.  The items (direct and mirror windows of the
+// samples, see F3) whose 9x9 window intersects the tile are listed sorted by
+// window row.  A chunk of them is staged in LDS once for the SM_ZW slices:
+// 16-wide weight rows wyr[s][r] = wy[r-ly0], wxr[s][i] = wx[i-lx0] (zero outside
+// the window) and, per slice, P[s][col] = conj(cf[k][kind][p]) * q' (q' = q
+// direct, conj(q) mirror) with the real part in column p and the imaginary part
+// in column 8*NB + p.  The result is the Hermitian-symmetrised pair grid whose
+// inverse transform is f_{2p} + i f_{2p+1}.  For its row the wave takes the run
+// of items covering it and accumulates
 //   D[cell i][col] += sum_k wyr[k][r] * wxr[k][i] * P[k][col]
-// with v_mfma_f32_16x16x4f32 (16 cells x 16 columns x 4 items per
-// instruction).  No atomics; fixed summation order.
-constexpr int SM_CH = 128;  // samples staged per chunk (8 staging threads each)
-constexpr int SM_ZW = 2;    // slices per workgroup
-template <int NB>           // 16-column blocks: 2*MP <= 16*NB
-__global__ void __launch_bounds__(1024)
+// with v_mfma_f32_16x16x4f32 (16 cells x 16 columns x 4 items per instruction);
+// items of the run that do not cover the row have zero weight, and staging
+// slots past the chunk are zero, so the run is padded to whole MFMA steps
+// without tests.  Lane (li, lk) ends with cells 4lk..4lk+3 of column li and
+// writes them straight to HBM.  No atomics; fixed summation order.
+constexpr int SM_CH = 128;  // items staged per chunk (8 staging threads each)
+constexpr int SM_ZW = 4;    // slices per workgroup
+template <int NB>           // 16-column blocks: MP <= 8*NB (NB = 1 or 2)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))  // two workgroups per CU
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
           const int *__restrict__ sids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf, int n,
           int nt, int MP, float beta, int ntiles, int nzb) {
+    static_assert(NB == 1 || NB == 2, "one or two column blocks");
     constexpr int NC = 16 * NB;
-    constexpr int NCP = NC + 1;  // padded row of P
+    constexpr int HALF = 8 * NB;       // column of the imaginary part of pair 0
+    constexpr int NCP = NC + 1;        // padded row of P
     constexpr int NBIN = TS + KW - 1;  // window rows ly0 in [-8, TS)
+    constexpr int NPP = HALF / 8;      // pairs per staging thread
     __shared__ float wyr[SM_CH][TS];
     __shared__ float wxr[SM_CH][TS];
     __shared__ float sP[SM_ZW][SM_CH][NCP];
     __shared__ int sly[SM_CH];
     __shared__ int bfirst[NBIN], blast[NBIN];
-    static_assert(SM_ZW * SM_CH * NCP >= 16 * 16 * NC, "output transpose must fit in the P buffer");
     const int tid = threadIdx.x;
     const int N = 2 * n, nT = N / TS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -430,7 +435,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
     const int r = tid >> 6, l = tid & 63;
     const int li = l & 15, lk = l >> 4;
-    const int sm = tid >> 3, part = tid & 7;  // staging: sample, eighth
+    const int sm = tid >> 3, part = tid & 7;  // staging: item slot, eighth
     v4f acc[SM_ZW][NB];
 #pragma unroll
     for (int zz = 0; zz < SM_ZW; zz++)
@@ -440,7 +445,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
     // dependent global loads (item -> geometry, q, coefficients) overlap the MFMA loop
     int pid = -1;
     SampleGeom pg = {0, 0, 0.f, 0.f};
-    float2 pq[SM_ZW], pph[(NC / 2 + 7) / 8];
+    float2 pq[SM_ZW], pph[NPP];
     auto prefetch = [&](int c0) {
         const int cnt = min(SM_CH, te.count - c0);
         pid = -1;
@@ -457,7 +462,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
             }
             const float2 *ph = cf + ((size_t)(sid / n) * 2 + (pid & 1)) * MP;
 #pragma unroll
-            for (int q = 0; q < (NC / 2 + 7) / 8; q++) {
+            for (int q = 0; q < NPP; q++) {
                 const int j = part + 8 * q;
                 pph[q] = j < MP ? ph[j] : make_float2(0.f, 0.f);
             }
@@ -489,22 +494,30 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
                 wyr[sm][i] = ((unsigned)ay < (unsigned)KW) ? expf(-beta * dyv * dyv) : 0.f;
                 wxr[sm][i] = ((unsigned)ax < (unsigned)KW) ? expf(-beta * dxv * dxv) : 0.f;
             }
-            // P columns for p = part, part + 8, ... (and the zero padding)
 #pragma unroll
             for (int zz = 0; zz < SM_ZW; zz++) {
-                if (zz < zc) {
 #pragma unroll
-                    for (int q = 0; q < (NC / 2 + 7) / 8; q++) {
-                        const int j = part + 8 * q;
-                        if (j < MP) {
-                            const float2 p = cmulc(pq[zz], pph[q]);  // q' * conj(cf)
-                            sP[zz][sm][j] = p.x;
-                            sP[zz][sm][MP + j] = p.y;
-                        }
-                    }
-                    for (int c = 2 * MP + part; c < NC; c += 8) sP[zz][sm][c] = 0.f;
+                for (int q = 0; q < NPP; q++) {
+                    const int j = part + 8 * q;
+                    const float2 p = cmulc(pq[zz], pph[q]);  // q' * conj(cf); zero for j >= MP
+                    sP[zz][sm][j] = p.x;
+                    sP[zz][sm][HALF + j] = p.y;
                 }
             }
+        } else {
+            // empty slot: zero weights and values
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                wyr[sm][2 * part + h] = 0.f;
+                wxr[sm][2 * part + h] = 0.f;
+            }
+#pragma unroll
+            for (int zz = 0; zz < SM_ZW; zz++)
+#pragma unroll
+                for (int q = 0; q < NPP; q++) {
+                    sP[zz][sm][part + 8 * q] = 0.f;
+                    sP[zz][sm][HALF + part + 8 * q] = 0.f;
+                }
         }
         if (c0 + SM_CH < te.count) prefetch(c0 + SM_CH);
         __syncthreads();
@@ -515,42 +528,52 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
             if (tid == cnt - 1 || sly[tid + 1] != v) blast[v + KW - 1] = tid + 1;
         }
         __syncthreads();
-        // samples covering row r: bins ly0 = r-8 .. r
+        // items covering row r: bins ly0 = r-8 .. r; the run is widened to
+        // multiples of 4 (slots stay < SM_CH)
         int lo = cnt, hi = 0;
 #pragma unroll
         for (int q = 0; q < KW; q++) {
             lo = min(lo, bfirst[r + q]);
             hi = max(hi, blast[r + q]);
         }
+        lo &= ~3;
+        hi = (hi + 3) & ~3;
+        const float *pw = &wyr[lk][r];
+        const float *px = &wxr[lk][li];
+        const float *pp = &sP[0][lk][li];
+#pragma unroll 2
         for (int k0 = lo; k0 < hi; k0 += 4) {
-            const int s = k0 + lk;
-            const bool ok = s < hi;
-            const int sc = ok ? s : lo;
-            const float av = ok ? wyr[sc][r] * wxr[sc][li] : 0.f;
+            const float av = pw[k0 * TS] * px[k0 * TS];
 #pragma unroll
             for (int zz = 0; zz < SM_ZW; zz++) {
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
-                    const float bv = sP[zz][sc][b * 16 + li];
+                    const float bv = pp[(zz * SM_CH + k0) * NCP + b * 16];
                     acc[zz][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[zz][b], 0, 0, 0);
                 }
             }
         }
     }
-    // acc[zz][b][v] = D_zz[cell 4*lk + v][col 16*b + li]; transpose through LDS, write rows
-    float *wl = &sP[0][0][0] + (size_t)r * 16 * NC;  // [cell][col] of this wave's row
-    float2 *Gt = G + (size_t)(ty0 + r) * N + tx0;
-    for (int zz = 0; zz < zc; zz++) {
-        __syncthreads();
+    // acc[zz][b][v] = D_zz[cell 4*lk + v][col 16*b + li]: real part of pair p in
+    // column p, imaginary part in column HALF + p
+    float2 *Gt = G + (size_t)(ty0 + r) * N + tx0 + 4 * lk;
 #pragma unroll
-        for (int b = 0; b < NB; b++)
+    for (int zz = 0; zz < SM_ZW; zz++) {
+        float re[4], im[4];
 #pragma unroll
-            for (int v = 0; v < 4; v++) wl[(4 * lk + v) * NC + 16 * b + li] = acc[zz][b][v];
-        __syncthreads();
-        float2 *Gr = Gt + (size_t)(z0 + zz) * MP * N * N;
-        for (int idx = l; idx < MP * 16; idx += 64) {
-            const int j = idx >> 4, i = idx & 15;
-            Gr[(size_t)j * N * N + i] = make_float2(wl[i * NC + j], wl[i * NC + MP + j]);
+        for (int v = 0; v < 4; v++) {
+            if constexpr (NB == 1) {
+                re[v] = acc[zz][0][v];
+                im[v] = __shfl(acc[zz][0][v], (l & 48) | ((li + 8) & 15), 64);
+            } else {
+                re[v] = acc[zz][0][v];
+                im[v] = acc[zz][1][v];
+            }
+        }
+        if (zz < zc && li < MP) {
+            float4 *o = reinterpret_cast<float4 *>(Gt + ((size_t)(z0 + zz) * MP + li) * N * N);
+            o[0] = make_float4(re[0], im[0], re[1], im[1]);
+            o[1] = make_float4(re[2], im[2], re[3], im[3]);
         }
     }
 }

@@ -593,14 +593,12 @@ struct Solver {
         {
             const int blocks = na_tiles * ((zc + SM_ZW - 1) / SM_ZW);
             launch("scatter", [&] {
-                if (2 * MP <= 16)
+                // MP <= 16 (M <= 32): one or two 16-column blocks
+                if (MP <= 8)
                     hipLaunchKernelGGL(k_scatter<1>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
                                        d_atiles.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
-                else if (2 * MP <= 32)
-                    hipLaunchKernelGGL(k_scatter<2>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
-                                       d_atiles.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
                 else
-                    hipLaunchKernelGGL(k_scatter<4>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
+                    hipLaunchKernelGGL(k_scatter<2>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
                                        d_atiles.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
             });
         }
