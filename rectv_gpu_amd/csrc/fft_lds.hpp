@@ -321,6 +321,71 @@ __device__ __forceinline__ void fft_p2(float2 *buf, int stride, int T, const flo
     p2_stages<N, 1, E, INV>(buf + t * stride, t < T, jl, tw);
 }
 
+// Register-in / register-out power-of-two transform (compile-time N >= 64).
+// The TPT = N/16 threads of a transform each hold 16 elements: thread jl supplies
+// x[jl + m*TPT] in v[m] and receives X[jl + m*TPT] in v[m], m = 0..15.  The
+// first radix-16 stage (no twiddles) runs on the registers, the radix-16 stages
+// between exchange through the transform's LDS row, and the last stage (radix
+// N/Ns_last, 16/RL butterflies per thread) leaves its outputs in registers:
+// butterfly j = jl + b*TPT writes X[j + r*Ns_last] = X[jl + (b + MB*r)*TPT].
+// Every thread of the workgroup must call it (it synchronises); `active`
+// masks threads without a transform.  The caller synchronises before the call
+// when the row may still be read by a previous transform.
+template <int N>
+__host__ __device__ constexpr int p2_last_ns() {
+    int ns = 16;
+    while (N / ns > 16) ns *= 16;
+    return ns;
+}
+template <int N, int Ns, bool INV>
+__device__ __forceinline__ void p2_mid(float2 *row, bool active, int jl, const float2 *__restrict__ tw) {
+    if constexpr (N / Ns > 16) {
+        p2_stage<16, N, Ns, 16, INV>(row, active, jl, tw);
+        p2_mid<N, Ns * 16, INV>(row, active, jl, tw);
+    }
+}
+template <int N, bool INV>
+__device__ __forceinline__ void fft_p2_reg(float2 (&v)[16], float2 *row, bool active, int jl,
+                                           const float2 *__restrict__ tw) {
+    static_assert((N & (N - 1)) == 0 && N >= 64, "power-of-two length >= 64");
+    constexpr int TPT = N / 16;
+    constexpr int NS = p2_last_ns<N>();
+    constexpr int RL = N / NS;
+    constexpr int MB = 16 / RL;
+    if (active) {
+        dft16<INV>(v);
+        float2 *o = row + lp(16 * jl);  // outputs 16*jl + r: constant offsets r + r/16
+#pragma unroll
+        for (int r = 0; r < 16; r++) o[r] = v[r];
+    }
+    __syncthreads();
+    p2_mid<N, 16, INV>(row, active, jl, tw);
+    if (active) {
+        float2 u[MB][RL];
+#pragma unroll
+        for (int b = 0; b < MB; b++) {
+            const int j = jl + b * TPT;
+#pragma unroll
+            for (int r = 0; r < RL; r++) u[b][r] = row[lp(j + r * NS)];
+        }
+#pragma unroll
+        for (int b = 0; b < MB; b++) {
+            const int j = jl + b * TPT;
+#pragma unroll
+            for (int r = 1; r < RL; r++) {
+                float2 w = tw[j * r];
+                if (INV) w.y = -w.y;
+                u[b][r] = cmul(u[b][r], w);
+            }
+            dft<RL, INV>(u[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < MB; b++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) v[b + MB * r] = u[b][r];
+    }
+}
+
 // NP > 0: compile-time power-of-two length NP; NP == 0: runtime plan (radices
 // 2, 3, 4, 5, 7, 8).
 template <int NP, int E, bool INV>

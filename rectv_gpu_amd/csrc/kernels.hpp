@@ -72,36 +72,73 @@ __global__ void __launch_bounds__(FFT_THREADS)
 k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__restrict__ deap, FftPlan plan,
            const float2 *__restrict__ tw, int n, int M, int MP, int T, int stride) {
     extern __shared__ float2 lds[];
-    if constexpr (NP > 0) n = NP / 2;
-    if constexpr (NP > 0) T = fft_rows_p2(NP);
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int ngr = (MP + T - 1) / T;
-    const int pg = L % ngr;
-    const int y = (L / ngr) % n;
-    const int z = L / ngr / n;
-    const int p0 = pg * T;
-    const int Tc = min(T, MP - p0);
-    const int N = NP ? NP : 2 * n, h = N / 4;
-    const size_t plane = (size_t)n * n;
-    const float *fz = f + (size_t)z * M * plane + (size_t)y * n;
-    const float dy = deap[y];
-    for (int idx = tid; idx < Tc * n; idx += nthr) {
-        const int pt = idx / n, x = idx - pt * n;
-        const int i0 = 2 * (p0 + pt);
-        const float va = fz[(size_t)i0 * plane + x];
-        const float vb = i0 + 1 < M ? fz[(size_t)(i0 + 1) * plane + x] : 0.f;
-        const float sc = deap[x] * dy * sgn(x + h);
-        lds[pt * stride + lp(x + h)] = make_float2(va * sc, vb * sc);
-        const int q = x < h ? x : x + n;  // zero padding
-        lds[pt * stride + lp(q)] = make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fft_any<NP, FFT_E, false>(lds, stride, Tc, plan, tw, tid, nthr);
-    float2 *Hz = H + ((size_t)z * MP + p0) * n * N + (size_t)y * N;
-    for (int idx = tid; idx < Tc * N; idx += nthr) {
-        const int pt = idx / N, kx = idx - pt * N;
-        Hz[(size_t)pt * n * N + kx] = cscale(lds[pt * stride + lp(kx)], sgn(kx));
+    if constexpr (NP > 0) {
+        // thread (t, jl): pair p0 + t; element positions jl + m*TPT of the
+        // 2n-point row; the n image pixels sit at positions 4*TPT .. 12*TPT - 1
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_rows_p2(NP);
+        const int ngr = (MP + TT - 1) / TT;
+        const int pg = L % ngr;
+        const int y = (L / ngr) % nn;
+        const int z = L / ngr / nn;
+        const int t = tid / TPT, jl = tid % TPT;
+        const int p = pg * TT + t;
+        const bool active = p < MP;
+        const size_t plane = (size_t)nn * nn;
+        const float sg = sgn(jl);  // (-1)^position: TPT is even
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = make_float2(0.f, 0.f);
+        if (active) {
+            const int i0 = 2 * p;
+            const float *fa = f + ((size_t)z * M + i0) * plane + (size_t)y * nn + jl;
+            const float sy = deap[y] * sg;
+            float va[8], vb[8], dx[8];
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                va[m] = fa[m * TPT];
+                vb[m] = i0 + 1 < M ? fa[plane + m * TPT] : 0.f;
+                dx[m] = deap[jl + m * TPT];
+            }
+#pragma unroll
+            for (int m = 0; m < 8; m++) v[m + 4] = make_float2(va[m] * dx[m] * sy, vb[m] * dx[m] * sy);
+        }
+        fft_p2_reg<NP, false>(v, lds + t * stride, active, jl, tw + plan.tw_off);
+        if (active) {
+            float2 *Hr = H + (((size_t)z * MP + p) * nn + y) * N + jl;
+#pragma unroll
+            for (int m = 0; m < 16; m++) Hr[m * TPT] = cscale(v[m], sg);
+        }
+    } else {
+        const int nthr = blockDim.x;
+        const int ngr = (MP + T - 1) / T;
+        const int pg = L % ngr;
+        const int y = (L / ngr) % n;
+        const int z = L / ngr / n;
+        const int p0 = pg * T;
+        const int Tc = min(T, MP - p0);
+        const int N = 2 * n, h = N / 4;
+        const size_t plane = (size_t)n * n;
+        const float *fz = f + (size_t)z * M * plane + (size_t)y * n;
+        const float dy = deap[y];
+        for (int idx = tid; idx < Tc * n; idx += nthr) {
+            const int pt = idx / n, x = idx - pt * n;
+            const int i0 = 2 * (p0 + pt);
+            const float va = fz[(size_t)i0 * plane + x];
+            const float vb = i0 + 1 < M ? fz[(size_t)(i0 + 1) * plane + x] : 0.f;
+            const float sc = deap[x] * dy * sgn(x + h);
+            lds[pt * stride + lp(x + h)] = make_float2(va * sc, vb * sc);
+            const int q = x < h ? x : x + n;  // zero padding
+            lds[pt * stride + lp(q)] = make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        fft_lds<FFT_E, false, true>(lds, stride, Tc, plan, tw, tid, nthr);
+        float2 *Hz = H + ((size_t)z * MP + p0) * n * N + (size_t)y * N;
+        for (int idx = tid; idx < Tc * N; idx += nthr) {
+            const int pt = idx / N, kx = idx - pt * N;
+            Hz[(size_t)pt * n * N + kx] = cscale(lds[pt * stride + lp(kx)], sgn(kx));
+        }
     }
 }
 
@@ -113,28 +150,48 @@ __global__ void __launch_bounds__(FFT_THREADS)
 k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, const float2 *__restrict__ tw,
            int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
-    if constexpr (NP > 0) n = NP / 2;
-    if constexpr (NP > 0) T = fft_rows_p2(NP);
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int N = NP ? NP : 2 * n, h = N / 4;
+    const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int ntile = N / T;
-    const int kt = L % ntile;
-    const int zj = L / ntile;  // z*M + j
-    const int kx0 = kt * T;
-    const float2 *Hs = H + (size_t)zj * n * N + kx0;
-    for (int idx = tid; idx < T * n; idx += nthr) {
-        const int t = idx % T, y = idx / T;
-        lds[t * stride + lp(y + h)] = cscale(Hs[(size_t)y * N + t], sgn(y + h));
-        const int q = y < h ? y : y + n;
-        lds[t * stride + lp(q)] = make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fft_any<NP, FFT_E, false>(lds, stride, T, plan, tw, tid, nthr);
-    float2 *Gs = G + (size_t)zj * N * N + kx0;
-    for (int idx = tid; idx < T * N; idx += nthr) {
-        const int t = idx % T, ky = idx / T;
-        Gs[(size_t)ky * N + t] = cscale(lds[t * stride + lp(ky)], sgn(ky));
+    if constexpr (NP > 0) {
+        // thread (t, jl), t fastest: adjacent lanes read adjacent columns; the n
+        // input rows sit at positions 4*TPT .. 12*TPT - 1
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_rows_p2(NP);
+        const int ntile = N / TT;
+        const int kt = L % ntile;
+        const int zj = L / ntile;  // z*MP + p
+        const int t = tid % TT, jl = tid / TT;
+        const float sg = sgn(jl);
+        const float2 *Hs = H + (size_t)zj * nn * N + kt * TT + t + (size_t)jl * N;
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m + 4] = cscale(Hs[(size_t)m * TPT * N], sg);
+        fft_p2_reg<NP, false>(v, lds + t * stride, true, jl, tw + plan.tw_off);
+        float2 *Gs = G + (size_t)zj * N * N + kt * TT + t + (size_t)jl * N;
+#pragma unroll
+        for (int m = 0; m < 16; m++) Gs[(size_t)m * TPT * N] = cscale(v[m], sg);
+    } else {
+        const int nthr = blockDim.x;
+        const int N = 2 * n, h = N / 4;
+        const int ntile = N / T;
+        const int kt = L % ntile;
+        const int zj = L / ntile;  // z*MP + p
+        const int kx0 = kt * T;
+        const float2 *Hs = H + (size_t)zj * n * N + kx0;
+        for (int idx = tid; idx < T * n; idx += nthr) {
+            const int t = idx % T, y = idx / T;
+            lds[t * stride + lp(y + h)] = cscale(Hs[(size_t)y * N + t], sgn(y + h));
+            const int q = y < h ? y : y + n;
+            lds[t * stride + lp(q)] = make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        fft_lds<FFT_E, false, true>(lds, stride, T, plan, tw, tid, nthr);
+        float2 *Gs = G + (size_t)zj * N * N + kx0;
+        for (int idx = tid; idx < T * N; idx += nthr) {
+            const int t = idx % T, ky = idx / T;
+            Gs[(size_t)ky * N + t] = cscale(lds[t * stride + lp(ky)], sgn(ky));
+        }
     }
 }
 
@@ -328,13 +385,60 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
       const float2 *__restrict__ tw, int n, int nrays, int T, int stride, int mode,
       double *__restrict__ partial) {
     extern __shared__ float2 lds[];
-    if constexpr (NP > 0) n = NP;
-    if constexpr (NP > 0) T = fft_rows_p2(NP);
     __shared__ double red[FFT_THREADS / 64];
     const int tid = threadIdx.x, nthr = blockDim.x;
+    double acc = 0.0;
+    if constexpr (NP > 0) {
+        // thread (t, jl) of ray r0 + t holds detector positions s = jl + m*TPT,
+        // which are both the outputs of one transform and the inputs of the next
+        constexpr int nn = NP, TPT = NP / 16, TT = fft_rows_p2(NP);
+        const int t = tid / TPT, jl = tid % TPT;
+        const int ray = blockIdx.x * TT + t;
+        const bool active = ray < nrays;
+        float2 *row = lds + t * stride;
+        const float2 *twn = tw + plan.tw_off;
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = make_float2(0.f, 0.f);
+        const size_t o0 = (size_t)ray * nn + jl;
+        if (mode & RAY_FWD) {
+            if (active) {
+                const float4 *pq4 = reinterpret_cast<const float4 *>(gq) + o0;
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    const float4 pq = pq4[m * TPT];
+                    // slot pair of s == 0 is never written by the gather
+                    if (m > 0 || jl > 0) v[m] = cmul(make_float2(pq.x + pq.z, pq.y - pq.w), pre_f[jl + m * TPT]);
+                }
+            }
+            fft_p2_reg<NP, true>(v, row, active, jl, twn);
+            if (active) {
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    const int sd = jl + m * TPT;
+                    float val = v[m].x * post_f[sd];
+                    if (mode & RAY_SUBG) val -= g[o0 + m * TPT];
+                    acc += (double)val * val;
+                    if (h1out) h1out[o0 + m * TPT] = val;
+                    v[m] = make_float2(val * pre_a[sd], 0.f);
+                }
+            }
+        } else if (active) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = make_float2(h1in[o0 + m * TPT] * pre_a[jl + m * TPT], 0.f);
+        }
+        if (mode & RAY_ADJ) {
+            __syncthreads();  // the row is reused
+            fft_p2_reg<NP, false>(v, row, active, jl, twn);
+            if (active) {
+#pragma unroll
+                for (int m = 0; m < 16; m++)
+                    gq[(o0 + m * TPT) * 2] = (m > 0 || jl > 0) ? cmul(v[m], post_a[jl + m * TPT]) : make_float2(0.f, 0.f);
+            }
+        }
+    } else {
     const int r0 = blockIdx.x * T;
     const int Tc = min(T, nrays - r0);
-    double acc = 0.0;
     if (mode & RAY_FWD) {
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
@@ -346,7 +450,7 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
             lds[t * stride + lp(s)] = v;
         }
         __syncthreads();
-        fft_any<NP, FFT_E, true>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_lds<FFT_E, true, true>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
             const size_t o = (size_t)(r0 + t) * n + s;
@@ -364,13 +468,14 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
     }
     if (mode & RAY_ADJ) {
         __syncthreads();
-        fft_any<NP, FFT_E, false>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_lds<FFT_E, false, true>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
             float2 v = make_float2(0.f, 0.f);
             if (s != 0) v = cmul(lds[t * stride + lp(s)], post_a[s]);
             gq[((size_t)(r0 + t) * n + s) * 2] = v;
         }
+    }
     }
     if (partial) {
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
@@ -585,26 +690,43 @@ __global__ void __launch_bounds__(FFT_THREADS)
 k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, const float2 *__restrict__ tw,
            int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
-    if constexpr (NP > 0) n = NP / 2;
-    if constexpr (NP > 0) T = fft_rows_p2(NP);
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int N = NP ? NP : 2 * n, h = N / 4;
+    const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int ntile = N / T;
-    const int kt = L % ntile;
-    const int zj = L / ntile;
-    const int kx0 = kt * T;
-    const float2 *Gs = G + (size_t)zj * N * N + kx0;
-    for (int idx = tid; idx < T * N; idx += nthr) {
-        const int t = idx % T, ky = idx / T;
-        lds[t * stride + lp(ky)] = cscale(Gs[(size_t)ky * N + t], sgn(ky));
-    }
-    __syncthreads();
-    fft_any<NP, FFT_E, true>(lds, stride, T, plan, tw, tid, nthr);
-    float2 *Hs = H + (size_t)zj * n * N + kx0;
-    for (int idx = tid; idx < T * n; idx += nthr) {
-        const int t = idx % T, y = idx / T;
-        Hs[(size_t)y * N + t] = cscale(lds[t * stride + lp(y + h)], sgn(y + h));
+    if constexpr (NP > 0) {
+        // thread (t, jl), t fastest; output positions 4*TPT .. 12*TPT - 1 are kept
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_rows_p2(NP);
+        const int ntile = N / TT;
+        const int kt = L % ntile;
+        const int zj = L / ntile;
+        const int t = tid % TT, jl = tid / TT;
+        const float sg = sgn(jl);
+        const float2 *Gs = G + (size_t)zj * N * N + kt * TT + t + (size_t)jl * N;
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = cscale(Gs[(size_t)m * TPT * N], sg);
+        fft_p2_reg<NP, true>(v, lds + t * stride, true, jl, tw + plan.tw_off);
+        float2 *Hs = H + (size_t)zj * nn * N + kt * TT + t + (size_t)jl * N;
+#pragma unroll
+        for (int m = 4; m < 12; m++) Hs[(size_t)(m - 4) * TPT * N] = cscale(v[m], sg);
+    } else {
+        const int nthr = blockDim.x;
+        const int N = 2 * n, h = N / 4;
+        const int ntile = N / T;
+        const int kt = L % ntile;
+        const int zj = L / ntile;
+        const int kx0 = kt * T;
+        const float2 *Gs = G + (size_t)zj * N * N + kx0;
+        for (int idx = tid; idx < T * N; idx += nthr) {
+            const int t = idx % T, ky = idx / T;
+            lds[t * stride + lp(ky)] = cscale(Gs[(size_t)ky * N + t], sgn(ky));
+        }
+        __syncthreads();
+        fft_lds<FFT_E, true, true>(lds, stride, T, plan, tw, tid, nthr);
+        float2 *Hs = H + (size_t)zj * n * N + kx0;
+        for (int idx = tid; idx < T * n; idx += nthr) {
+            const int t = idx % T, y = idx / T;
+            Hs[(size_t)y * N + t] = cscale(lds[t * stride + lp(y + h)], sgn(y + h));
+        }
     }
 }
 
@@ -615,28 +737,46 @@ __global__ void __launch_bounds__(FFT_THREADS)
 k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__restrict__ deap, FftPlan plan,
            const float2 *__restrict__ tw, int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
-    if constexpr (NP > 0) n = NP / 2;
-    if constexpr (NP > 0) T = fft_rows_p2(NP);
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int N = NP ? NP : 2 * n, h = N / 4;
+    const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int nyt = (n + T - 1) / T;
-    const int yt = L % nyt;
-    const int zj = L / nyt;
-    const int y0 = yt * T;
-    const int Tc = min(T, n - y0);
-    const float2 *Hs = H + ((size_t)zj * n + y0) * N;
-    for (int idx = tid; idx < Tc * N; idx += nthr) {
-        const int t = idx / N, kx = idx - t * N;
-        lds[t * stride + lp(kx)] = cscale(Hs[(size_t)t * N + kx], sgn(kx));
-    }
-    __syncthreads();
-    fft_any<NP, FFT_E, true>(lds, stride, Tc, plan, tw, tid, nthr);
-    float2 *vs = v + ((size_t)zj * n + y0) * n;
-    for (int idx = tid; idx < Tc * n; idx += nthr) {
-        const int t = idx / n, x = idx - t * n;
-        const float s = sgn(x + h) * deap[x] * deap[y0 + t];
-        vs[(size_t)t * n + x] = cscale(lds[t * stride + lp(x + h)], s);
+    if constexpr (NP > 0) {
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_rows_p2(NP);
+        constexpr int nyt = nn / TT;
+        const int yt = L % nyt;
+        const int zj = L / nyt;
+        const int t = tid / TPT, jl = tid % TPT;
+        const int y = yt * TT + t;
+        const float sg = sgn(jl);
+        const float2 *Hr = H + ((size_t)zj * nn + y) * N + jl;
+        float2 u[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) u[m] = cscale(Hr[m * TPT], sg);
+        fft_p2_reg<NP, true>(u, lds + t * stride, true, jl, tw + plan.tw_off);
+        const float sy = sg * deap[y];
+        float2 *vr = v + ((size_t)zj * nn + y) * nn + jl;
+#pragma unroll
+        for (int m = 4; m < 12; m++) vr[(m - 4) * TPT] = cscale(u[m], sy * deap[jl + (m - 4) * TPT]);
+    } else {
+        const int nthr = blockDim.x;
+        const int N = 2 * n, h = N / 4;
+        const int nyt = (n + T - 1) / T;
+        const int yt = L % nyt;
+        const int zj = L / nyt;
+        const int y0 = yt * T;
+        const int Tc = min(T, n - y0);
+        const float2 *Hs = H + ((size_t)zj * n + y0) * N;
+        for (int idx = tid; idx < Tc * N; idx += nthr) {
+            const int t = idx / N, kx = idx - t * N;
+            lds[t * stride + lp(kx)] = cscale(Hs[(size_t)t * N + kx], sgn(kx));
+        }
+        __syncthreads();
+        fft_lds<FFT_E, true, true>(lds, stride, Tc, plan, tw, tid, nthr);
+        float2 *vs = v + ((size_t)zj * n + y0) * n;
+        for (int idx = tid; idx < Tc * n; idx += nthr) {
+            const int t = idx / n, x = idx - t * n;
+            const float s = sgn(x + h) * deap[x] * deap[y0 + t];
+            vs[(size_t)t * n + x] = cscale(lds[t * stride + lp(x + h)], s);
+        }
     }
 }
 
