@@ -13,7 +13,8 @@
 //   f        [nz][M][n][n]                 f32   time frames of each slice
 //   H        [zb][MP][n][2n]               c64   row-transformed spectra (forward),
 //                                                 column-transformed (adjoint)
-//   G        [zb][MP][2n][2n]              c64   oversampled spectra, centred
+//   G        [zb][MP][2n][2n]              c64   oversampled spectra, centred, stored
+//                                                 transposed: G[kx][ky] (column FFT rows)
 //   gq       [zb][ntheta][n][2]            c64   per sample: direct / mirror gather
 //                                                 sums (forward); slot 0 = q (adjoint)
 //   v        [zb][MP][n][n]                c64   adjoint frame pairs (re = f_{2p}, im = f_{2p+1})
@@ -168,9 +169,10 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
 #pragma unroll
         for (int m = 0; m < 8; m++) v[m + 4] = cscale(Hs[(size_t)m * TPT * N], sg);
         fft_p2_reg<NP, false>(v, lds + t * stride, true, jl, tw + plan.tw_off);
-        float2 *Gs = G + (size_t)zj * N * N + kt * TT + t + (size_t)jl * N;
+        // transposed store: the transform of column kx is row kx of G
+        float2 *Gs = G + (size_t)zj * N * N + (size_t)(kt * TT + t) * N + jl;
 #pragma unroll
-        for (int m = 0; m < 16; m++) Gs[(size_t)m * TPT * N] = cscale(v[m], sg);
+        for (int m = 0; m < 16; m++) Gs[m * TPT] = cscale(v[m], sg);
     } else {
         const int nthr = blockDim.x;
         const int N = 2 * n, h = N / 4;
@@ -187,10 +189,10 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
         }
         __syncthreads();
         fft_lds<FFT_E, false, true>(lds, stride, T, plan, tw, tid, nthr);
-        float2 *Gs = G + (size_t)zj * N * N + kx0;
+        float2 *Gs = G + (size_t)zj * N * N + (size_t)kx0 * N;
         for (int idx = tid; idx < T * N; idx += nthr) {
-            const int t = idx % T, ky = idx / T;
-            Gs[(size_t)ky * N + t] = cscale(lds[t * stride + lp(ky)], sgn(ky));
+            const int t = idx / N, ky = idx - t * N;
+            Gs[(size_t)t * N + ky] = cscale(lds[t * stride + lp(ky)], sgn(ky));
         }
     }
 }
@@ -700,10 +702,10 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
         const int zj = L / ntile;
         const int t = tid % TT, jl = tid / TT;
         const float sg = sgn(jl);
-        const float2 *Gs = G + (size_t)zj * N * N + kt * TT + t + (size_t)jl * N;
+        const float2 *Gs = G + (size_t)zj * N * N + (size_t)(kt * TT + t) * N + jl;  // row kx of G
         float2 v[16];
 #pragma unroll
-        for (int m = 0; m < 16; m++) v[m] = cscale(Gs[(size_t)m * TPT * N], sg);
+        for (int m = 0; m < 16; m++) v[m] = cscale(Gs[m * TPT], sg);
         fft_p2_reg<NP, true>(v, lds + t * stride, true, jl, tw + plan.tw_off);
         float2 *Hs = H + (size_t)zj * nn * N + kt * TT + t + (size_t)jl * N;
 #pragma unroll
@@ -715,10 +717,10 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
         const int kt = L % ntile;
         const int zj = L / ntile;
         const int kx0 = kt * T;
-        const float2 *Gs = G + (size_t)zj * N * N + kx0;
+        const float2 *Gs = G + (size_t)zj * N * N + (size_t)kx0 * N;
         for (int idx = tid; idx < T * N; idx += nthr) {
-            const int t = idx % T, ky = idx / T;
-            lds[t * stride + lp(ky)] = cscale(Gs[(size_t)ky * N + t], sgn(ky));
+            const int t = idx / N, ky = idx - t * N;
+            lds[t * stride + lp(ky)] = cscale(Gs[(size_t)t * N + ky], sgn(ky));
         }
         __syncthreads();
         fft_lds<FFT_E, true, true>(lds, stride, T, plan, tw, tid, nthr);

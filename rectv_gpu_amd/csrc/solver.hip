@@ -101,11 +101,13 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             if (y0 >= 0.5) y0 = 0.5 - 1e-5;
             const double X = 2.0 * n * x0, Y = 2.0 * n * y0;
             const double fxl = std::floor(X), fyl = std::floor(Y);
+            // the oversampled spectra are stored transposed (G[kx][ky]): the window
+            // "row" runs along kx (X) and the "column" along ky (Y)
             SampleGeom g;
-            g.col0 = (int)(((long long)n + (long long)fxl - K) % N + N) % N;
-            g.row0 = (int)(((long long)n + (long long)fyl - K) % N + N) % N;
-            g.fx = (float)(X - fxl);
-            g.fy = (float)(Y - fyl);
+            g.row0 = (int)(((long long)n + (long long)fxl - K) % N + N) % N;
+            g.col0 = (int)(((long long)n + (long long)fyl - K) % N + N) % N;
+            g.fy = (float)(X - fxl);
+            g.fx = (float)(Y - fyl);
             // mirror: cells (2n - c) mod 2n of the window, taps reversed, fractions negated
             SampleGeom m;
             m.row0 = (2 * N - g.row0 - (KW - 1)) % N;
