@@ -235,6 +235,7 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
     __shared__ float2 acc[GMAX][GSZ];
     __shared__ int gofs[GMAX];
     __shared__ int gid[GMAX][GSZ];
+    __shared__ int gcf[GMAX][GSZ];  // coefficient row (k*2 + kind)*MP of each member
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int N = 2 * n, nT = N / TS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -250,6 +251,7 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
         const int4 gr = groups[te.start + g];
         const int id = q == 0 ? gr.x : q == 1 ? gr.y : q == 2 ? gr.z : gr.w;
         gid[g][q] = id;
+        gcf[g][q] = id >= 0 ? ((id >> 1) / n * 2 + (id & 1)) * MP : 0;
         acc[g][q] = make_float2(0.f, 0.f);
         if (q == 0) {
             const SampleGeom g0 = geo[gr.x];
@@ -307,10 +309,13 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
             const int it = it0 + tid;
             const int g = it / GJC, jl = it - (it / GJC) * GJC;
             const bool live = it < cnt * GJC && jl < Jc;
-            float2 c[GSZ];
+            float2 c[GSZ], cfv[GSZ];
 #pragma unroll
             for (int q = 0; q < GSZ; q++) c[q] = make_float2(0.f, 0.f);
             if (live) {
+                // coefficients first: their loads overlap the window sums
+#pragma unroll
+                for (int q = 0; q < GSZ; q++) cfv[q] = cf[gcf[g][q] + jc + jl];
                 float wx[GSZ][KW];
 #pragma unroll
                 for (int q = 0; q < GSZ; q++)
@@ -341,11 +346,9 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
                         sum[q].y += wy[q] * rs.y;
                     }
                 }
+                // empty member slots have zero weights (sum = 0) and coefficient row 0
 #pragma unroll
-                for (int q = 0; q < GSZ; q++) {
-                    const int id = gid[g][q];
-                    if (id >= 0) c[q] = cmul(cf[((size_t)((id >> 1) / n) * 2 + (id & 1)) * MP + jc + jl], sum[q]);
-                }
+                for (int q = 0; q < GSZ; q++) c[q] = cmul(cfv[q], sum[q]);
             }
             // sum the GJC basis functions of each group (lanes g*GJC .. g*GJC+GJC-1)
 #pragma unroll
