@@ -40,6 +40,8 @@ constexpr int TP = TH + 1;      // LDS row pitch of the gather tile
 
 __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 
+typedef float v4f __attribute__((ext_vector_type(4)));  // MFMA accumulator fragment
+
 // Transforms per FFT workgroup for a compile-time power-of-two length L:
 // FFT_WG / (L / FFT_E) rows, at most 16 (the host uses the same rule).  FFT_WG
 // = 512 threads keeps LDS near 70 KB so that two workgroups share a CU and one
@@ -373,6 +375,210 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
     }
 }
 
+// F3 on the matrix cores.  One workgroup per (tile entry of <= GM_ITEMS items
+// whose window origin lies in the tile, group of GM_ZG slices); items are sorted
+// by local origin (row, column) and packed by the host into 16-item blocks (one
+// wave each) spanning at most 12 rows and 4 steps of 4 cells.  Item weights are
+// staged once as 24-wide rows that are zero outside the window; per slice the
+// tile plus its 8-cell halo of all MP pair spectra is staged as
+// tile[row][cell][col], col = p (real part) and HALF + p (imaginary part), the
+// next slice's tile being loaded into registers meanwhile.  Per window row r of
+// its block and per 4 cells, a wave runs
+//   D[col][item] += sum_c tile[r][c][col] * wyr[item][r] * wxr[item][c]
+// as v_mfma_f32_16x16x4f32 (16 columns x 16 items x 4 cells); each lane then
+// holds 4 columns of one item, contracts them with cf (alpha_p[k] direct,
+// conj(beta_p[k]) mirror) and two cross-lane steps finish the sum over pairs.
+constexpr int GM_ITEMS = 128;  // items per workgroup: 8 waves x 16
+constexpr int GM_KS = 4;       // max steps of 4 cells per block (the host packs blocks so)
+constexpr int GM_ZG = 8;       // slices per workgroup
+template <int NB>              // 16-column blocks: MP <= 8*NB
+__global__ void __launch_bounds__(512)
+k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
+              const int *__restrict__ items, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf,
+              int n, int nt, int MP, float beta, int nentries, int nzb) {
+    static_assert(NB == 1 || NB == 2, "one or two column blocks");
+    constexpr int NC = 16 * NB, HALF = 8 * NB, NCP = NC + 1;
+    constexpr int CELLS = TH * TH;
+    constexpr int NLD = (HALF * CELLS + 511) / 512;  // tile loads per thread and slice
+    // 4 spare cells: the last 4-cell step of row TH-1 may read up to 3 cells past
+    // the region (their weights are zero; the spare cells are zeroed)
+    __shared__ float tile[(CELLS + 4) * NCP];
+    __shared__ float wyr[GM_ITEMS][TH];
+    __shared__ float wxr[GM_ITEMS][TH];
+    __shared__ int sid[GM_ITEMS];
+    __shared__ int sly[GM_ITEMS], slx[GM_ITEMS];
+    const int tid = threadIdx.x;
+    const int N = 2 * n, nT = N / TS;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int e = L % nentries;
+    const int z0 = (L / nentries) * GM_ZG;
+    const int zc = min(GM_ZG, nzb - z0);
+    const TileEntry te = tiles[e];
+    const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
+    const int cnt = te.count;
+    // per-thread tile offsets within one slice's spectra (all pairs of a slice
+    // span MP*4n^2 < 2^31 elements)
+    int tofs[NLD];
+#pragma unroll
+    for (int q = 0; q < NLD; q++) {
+        const int idx = tid + q * 512;
+        const int p = idx / CELLS, rc = idx - p * CELLS;
+        const int r = rc / TH, c = rc - r * TH;
+        int gy = ty0 + r;
+        if (gy >= N) gy -= N;
+        int gx = tx0 + c;
+        if (gx >= N) gx -= N;
+        tofs[q] = (idx < HALF * CELLS && p < MP) ? (p * N + gy) * N + gx : -1;
+    }
+    float2 stv[NLD];
+    auto load_tile = [&](int z) {
+        const float2 *Gz = G + (size_t)z * MP * N * N;
+#pragma unroll
+        for (int q = 0; q < NLD; q++) stv[q] = tofs[q] >= 0 ? Gz[tofs[q]] : make_float2(0.f, 0.f);
+    };
+    auto store_tile = [&] {
+#pragma unroll
+        for (int q = 0; q < NLD; q++) {
+            const int idx = tid + q * 512;
+            if (idx < HALF * CELLS) {
+                const int p = idx / CELLS, rc = idx - p * CELLS;
+                tile[rc * NCP + p] = stv[q].x;
+                tile[rc * NCP + HALF + p] = stv[q].y;
+            }
+        }
+    };
+    load_tile(z0);
+    // stage the items: 4 threads per item, 6 of the 24 weight entries each
+    {
+        const int it = tid >> 2, part = tid & 3;
+        const int id = it < cnt ? items[te.start + it] : -1;
+        SampleGeom g = {0, 0, 0.f, 0.f};
+        int ly0 = 64, lx0 = 64;  // empty slot: window outside the region
+        if (id >= 0) {
+            g = geo[id];
+            ly0 = g.row0 - ty0;
+            if (ly0 < 0) ly0 += N;
+            lx0 = g.col0 - tx0;
+            if (lx0 < 0) lx0 += N;
+        }
+        if (part == 0) {
+            sid[it] = id;
+            sly[it] = ly0;
+            slx[it] = lx0;
+        }
+#pragma unroll
+        for (int h = 0; h < TH / 4; h++) {
+            const int r = part * (TH / 4) + h;
+            const int ay = r - ly0, ax = r - lx0;
+            const float dyv = (float)(ay - 4) - g.fy, dxv = (float)(ax - 4) - g.fx;
+            wyr[it][r] = ((unsigned)ay < (unsigned)KW) ? expf(-beta * dyv * dyv) : 0.f;
+            wxr[it][r] = ((unsigned)ax < (unsigned)KW) ? expf(-beta * dxv * dxv) : 0.f;
+        }
+        if (tid < 4 * NCP) tile[CELLS * NCP + tid] = 0.f;
+    }
+    __syncthreads();
+    const int w = tid >> 6, l = tid & 63;
+    const int li = l & 15, lk = l >> 4;
+    const int b0 = w * 16;
+    // block window: rows [rlo, rhi], 4-cell steps from clo (padding slots do not count)
+    int rlo = 64, rhi = -1, clo = 64, chi = -1;
+#pragma unroll 4
+    for (int i = 0; i < 16; i++) {
+        if (b0 + i < cnt && sid[b0 + i] >= 0) {
+            rlo = min(rlo, sly[b0 + i]);
+            rhi = max(rhi, sly[b0 + i] + KW - 1);
+            clo = min(clo, slx[b0 + i]);
+            chi = max(chi, slx[b0 + i] + KW - 1);
+        }
+    }
+    // the bounds are equal across the wave: keep them in scalar registers
+    rlo = __builtin_amdgcn_readfirstlane(rlo);
+    rhi = __builtin_amdgcn_readfirstlane(rhi);  // -1: no item in this wave's block
+    clo = __builtin_amdgcn_readfirstlane(clo) & ~3;
+    const int nks = __builtin_amdgcn_readfirstlane((chi - clo) / 4 + 1);  // 3 or 4
+    // this lane's item (column li of D) and its coefficients for columns 4*lk + v
+    const int oid = b0 + li < cnt ? sid[b0 + li] : -1;
+    float2 cfv[NB][4];
+    {
+        const float2 *crow = cf + (oid >= 0 ? ((size_t)((oid >> 1) / n) * 2 + (oid & 1)) * MP : 0);
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int pp = (16 * b + 4 * lk + v) % HALF;
+                cfv[b][v] = (oid >= 0 && pp < MP) ? crow[pp] : make_float2(0.f, 0.f);
+            }
+    }
+    float wxv[GM_KS];
+#pragma unroll
+    for (int ks = 0; ks < GM_KS; ks++) {
+        const int c = clo + 4 * ks + lk;
+        wxv[ks] = (ks < nks && c < TH && rhi >= 0) ? wxr[b0 + li][c] : 0.f;
+    }
+    float2 *out = gq + (size_t)z0 * nt * n * 2;
+    for (int zz = 0; zz < zc; zz++) {
+        if (zz > 0) __syncthreads();  // previous slice's tile fully read
+        store_tile();
+        if (zz + 1 < zc) load_tile(z0 + zz + 1);
+        __syncthreads();
+        if (rhi >= 0) {
+            v4f acc0[NB], acc1[NB];
+#pragma unroll
+            for (int b = 0; b < NB; b++) acc0[b] = acc1[b] = (v4f){0.f, 0.f, 0.f, 0.f};
+            // a 9-cell window spans exactly 3 steps of 4 cells; blocks span 3 or 4
+            for (int r = rlo; r <= rhi; r++) {
+                const float wy = wyr[b0 + li][r];
+                const float *trow = tile + (r * TH + clo + lk) * NCP + li;
+                float bv[GM_KS][NB];
+#pragma unroll
+                for (int ks = 0; ks < 3; ks++)
+#pragma unroll
+                    for (int b = 0; b < NB; b++) bv[ks][b] = trow[4 * ks * NCP + 16 * b];
+                if (nks > 3) {
+#pragma unroll
+                    for (int b = 0; b < NB; b++) bv[3][b] = trow[12 * NCP + 16 * b];
+                }
+#pragma unroll
+                for (int ks = 0; ks < 3; ks++) {
+                    const float av = wy * wxv[ks];
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+                        if (ks & 1) acc1[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[ks][b], av, acc1[b], 0, 0, 0);
+                        else acc0[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[ks][b], av, acc0[b], 0, 0, 0);
+                    }
+                }
+                if (nks > 3) {
+                    const float av = wy * wxv[3];
+#pragma unroll
+                    for (int b = 0; b < NB; b++)
+                        acc1[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[3][b], av, acc1[b], 0, 0, 0);
+                }
+            }
+            // lane (li, lk): D[16*b + 4*lk + v][li]; column c is pair c % HALF, imaginary
+            // part for c >= HALF
+            float sx = 0.f, sy = 0.f;
+#pragma unroll
+            for (int b = 0; b < NB; b++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const float d = acc0[b][v] + acc1[b][v];
+                    if (16 * b + 4 * lk + v < HALF) {
+                        sx += cfv[b][v].x * d;
+                        sy += cfv[b][v].y * d;
+                    } else {  // cf * (i d)
+                        sx -= cfv[b][v].y * d;
+                        sy += cfv[b][v].x * d;
+                    }
+                }
+            sx += __shfl_xor(sx, 16, 64);
+            sy += __shfl_xor(sy, 16, 64);
+            sx += __shfl_xor(sx, 32, 64);
+            sy += __shfl_xor(sy, 32, 64);
+            if (lk == 0 && oid >= 0) out[(size_t)zz * nt * n * 2 + oid] = make_float2(sx, sy);
+        }
+    }
+}
+
 // F4: per-ray 1D transforms (one workgroup per T rays).
 //   FWD: h1[s'] = Re{post_f[s'] * IFFT(pre_f[s] * (P[s] + conj Q[s]))} (- g), with P, Q
 //        the direct and mirror gather sums of sample s   (shift, fftshift,
@@ -495,8 +701,6 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
 }
 
 constexpr int GS = 4;  // sample slots per origin group (int4 group records)
-
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 // A1: tile-owned Gaussian scatter on the FP32 matrix cores (replaces gather ADJ
 // with global atomicAdd, kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62,

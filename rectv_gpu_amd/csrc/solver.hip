@@ -61,8 +61,9 @@ static bool make_plan(int N, FftPlan &p) {
 struct HostGeom {
     std::vector<SampleGeom> geo;
     std::vector<int4> fgroups;
+    std::vector<int> fitems;  // forward items per origin tile for the matrix-core gather (-1 = padding)
     std::vector<int> asids;
-    std::vector<TileEntry> ftiles, atiles;
+    std::vector<TileEntry> ftiles, fmtiles, atiles;
     std::vector<float2> pre_f, post_a, cf;
     std::vector<float> post_f, pre_a, deap;
     float beta = 0.f;
@@ -181,6 +182,70 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         const double share = origins ? (double)keys.size() / origins : 1.0;
         hg.fgs = share >= 3.0 ? 4 : (share >= 1.6 ? 2 : 1);
         make_groups(keys, hg.fgroups, hg.ftiles, false, chunk_max, hg.fgs);
+        // flat lists for the matrix-core gather: items in origin raster order,
+        // packed into 16-item blocks (one wave each); a block is closed early
+        // when the next origin run would widen its window beyond 12 rows or 4
+        // steps of 4 cells; entries hold at most GM_ITEMS items
+        hg.fitems.clear();
+        hg.fmtiles.clear();
+        size_t a = 0;
+        while (a < keys.size()) {
+            const int t = (int)(keys[a].k >> 32);
+            const int ty0 = (t / nT) * TS, tx0 = (t % nT) * TS;
+            const int e0 = (int)hg.fitems.size();
+            int bcnt = 0, rlo = 0, rhi = 0, clo = 0, chi = 0;
+            auto close_block = [&] {
+                while (bcnt > 0 && bcnt < 16) {
+                    hg.fitems.push_back(-1);
+                    bcnt++;
+                }
+                bcnt = 0;
+            };
+            while (a < keys.size() && (int)(keys[a].k >> 32) == t) {
+                size_t b = a;
+                while (b < keys.size() && keys[b].k == keys[a].k) b++;
+                const SampleGeom &g = hg.geo[keys[a].id];
+                const int ly0 = (g.row0 - ty0 + N) % N, lx0 = (g.col0 - tx0 + N) % N;
+                for (size_t c = a; c < b; c++) {
+                    if (bcnt > 0) {
+                        const int nrlo = std::min(rlo, ly0), nrhi = std::max(rhi, ly0);
+                        const int nclo = std::min(clo, lx0) & ~3, nchi = std::max(chi, lx0) + KW - 1;
+                        if (nrhi - nrlo + KW > 12 || (nchi - nclo) / 4 + 1 > 4) close_block();
+                    }
+                    if (bcnt == 0) {
+                        rlo = rhi = ly0;
+                        clo = chi = lx0;
+                    }
+                    rlo = std::min(rlo, ly0);
+                    rhi = std::max(rhi, ly0);
+                    clo = std::min(clo, lx0);
+                    chi = std::max(chi, lx0);
+                    hg.fitems.push_back(keys[c].id);
+                    if (++bcnt == 16) bcnt = 0;
+                }
+                a = b;
+            }
+            close_block();
+            const int e1 = (int)hg.fitems.size();
+            // the kernel relies on every block spanning at most 12 rows and 4 steps of 4 cells
+            for (int b0 = e0; b0 < e1; b0 += 16) {
+                int r0 = 1 << 30, r1 = -1, c0 = 1 << 30, c1 = -1;
+                for (int i = b0; i < b0 + 16; i++) {
+                    if (hg.fitems[i] < 0) continue;
+                    const SampleGeom &g = hg.geo[hg.fitems[i]];
+                    const int ly0 = (g.row0 - ty0 + N) % N, lx0 = (g.col0 - tx0 + N) % N;
+                    r0 = std::min(r0, ly0);
+                    r1 = std::max(r1, ly0);
+                    c0 = std::min(c0, lx0);
+                    c1 = std::max(c1, lx0);
+                }
+                if (r1 < 0) continue;
+                if (r1 - r0 + KW > 12 || (c1 + KW - 1 - (c0 & ~3)) / 4 + 1 > GM_KS || r1 >= TS || c1 >= TS)
+                    throw Error(-1, "internal: gather block exceeds its window bounds");
+            }
+            for (int s0 = e0; s0 < e1; s0 += GM_ITEMS)
+                hg.fmtiles.push_back(TileEntry{t, s0, std::min(GM_ITEMS, e1 - s0), 0});
+        }
     }
     // adjoint lists: every tile the 9x9 window intersects, each tile's items
     // sorted by (local window row, local window column, id)
@@ -330,11 +395,13 @@ struct Solver {
     bool inputs = false;
     DBuf<SampleGeom> d_geo;
     DBuf<int4> d_fgroups;
+    DBuf<int> d_fitems;
     DBuf<int> d_asids;
-    DBuf<TileEntry> d_ftiles, d_atiles;
+    DBuf<TileEntry> d_ftiles, d_fmtiles, d_atiles;
     DBuf<float2> d_cf, d_pre_f, d_post_a;
     DBuf<float> d_post_f, d_pre_a, d_deap;
-    int nf_entries = 0, na_tiles = 0, fgs = 4;
+    int nf_entries = 0, nfm_entries = 0, na_tiles = 0, fgs = 4;
+    bool gather_valu = false;  // RECTV_GATHER=valu selects the VALU gather
     float beta = 0.f;
     // profiling
     bool prof = false;
@@ -512,6 +579,15 @@ struct Solver {
         upload(d_geo, hg.geo, bytes, st);
         upload(d_fgroups, hg.fgroups, bytes, st);
         upload(d_ftiles, hg.ftiles, bytes, st);
+        upload(d_fitems, hg.fitems, bytes, st);
+        upload(d_fmtiles, hg.fmtiles, bytes, st);
+        nfm_entries = (int)hg.fmtiles.size();
+        if (std::getenv("RECTV_VERBOSE")) {
+            size_t real = 0;
+            for (int v : hg.fitems) real += v >= 0;
+            fprintf(stderr, "rectv: gather items %zu in %zu slots, %d entries\n", real, hg.fitems.size(), nfm_entries);
+        }
+        gather_valu = std::getenv("RECTV_GATHER") && std::string(std::getenv("RECTV_GATHER")) == "valu";
         upload(d_asids, hg.asids, bytes, st);
         upload(d_atiles, hg.atiles, bytes, st);
         upload(d_cf, hg.cf, bytes, st);
@@ -562,6 +638,18 @@ struct Solver {
             });
         }
         {
+            if (!gather_valu) {
+                const int blocks = nfm_entries * ((zc + GM_ZG - 1) / GM_ZG);
+                if (blocks > 0)
+                    launch("gather", [&] {
+                        if (MP <= 8)
+                            hipLaunchKernelGGL(k_gather_mfma<1>, dim3(blocks), dim3(512), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                               d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc);
+                        else
+                            hipLaunchKernelGGL(k_gather_mfma<2>, dim3(blocks), dim3(512), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                               d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc);
+                    });
+            } else {
             const int blocks = nf_entries * zc;
             if (blocks > 0)
                 launch("gather", [&] {
@@ -575,6 +663,7 @@ struct Solver {
                         hipLaunchKernelGGL(k_gather<1>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
                                            d_fgroups.p, d_ftiles.p, d_cf.p, n, nt, MP, beta, nf_entries);
                 });
+            }
         }
         ray(zc, gdat, nullptr, h1out, mode, partial);
     }
@@ -871,6 +960,8 @@ struct Solver {
         d_fgroups.release(t);
         d_asids.release(t);
         d_ftiles.release(t);
+        d_fitems.release(t);
+        d_fmtiles.release(t);
         d_atiles.release(t);
         d_cf.release(t);
         d_pre_f.release(t);
