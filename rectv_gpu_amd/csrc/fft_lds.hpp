@@ -23,8 +23,12 @@ struct FftPlan {
 // LDS position of element i of a transform row: one pad slot per 16 elements
 // breaks the power-of-two strides of the Stockham stores (bank conflicts).
 __host__ __device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
-// Row pitch (in float2) of a padded transform of length N.
-__host__ __device__ __forceinline__ int lp_stride(int N) { return N + (N >> 4) + 2; }
+// Row pitch (in float2) of a padded transform of length N: = 4 (mod 16), so that
+// 4 transforms read by one 16-lane group (column passes) fall on disjoint banks.
+__host__ __device__ __forceinline__ int lp_stride(int N) {
+    const int s = N + (N >> 4);
+    return s + ((4 - s) & 15);
+}
 
 template <bool INV>
 __device__ inline float2 mul_mi(float2 a) {  // a * (-i) forward, a * (+i) inverse
