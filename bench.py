@@ -39,7 +39,8 @@ def kernel_bytes(name, n, nt, m, zc, nzp):
         "scatter": Gs * 8 + S * N * N * 8,
         "adj_cols": S * (N * N + n * N) * 8,
         "adj_rows": S * (n * N + n * n) * 8,
-        "update": V * (4 + 4 + 4) + 32 * V4,
+        "update": V * (4 + 4 + 4 + 4),         # f in, div(chi) in, adjoint frames in, f out
+        "divchi": 32 * V4 + 4 * V,             # psi, mu in; div(mu - rho psi) out
         "prox": 8 * V + 48 * V4,
     }.get(name)
 

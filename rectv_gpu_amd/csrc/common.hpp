@@ -47,4 +47,24 @@ __device__ inline int xcd_remap(int p, int total) {
     return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
+// Raw buffer access: a wave-uniform base in a buffer descriptor (SGPRs), a
+// 32-bit per-lane byte offset and a wave-uniform byte offset.  Keeps per-lane
+// address state at one VGPR where a loop walks many uniform sub-arrays.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p) {
+    // dword3 0x00020000: untyped 32-bit access on gfx9-family parts; no range limit
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, -1, 0x00020000);
+}
+// (the b32 / b64 builtins move raw bits: reinterpret, never convert)
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+// (two 32-bit loads: the 64-bit raw buffer load builtin of this toolchain
+// reads only the first dword)
+__device__ __forceinline__ float2 buf_ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return make_float2(buf_ld(r, voff, soff), buf_ld(r, voff + 4, soff));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
 }  // namespace rtv

@@ -1022,90 +1022,128 @@ __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i
     return ((((size_t)c * (a.nzp + 1) + zl) * (a.M + 1) + i) * (a.n + 1) + y) * (a.n + 1) + x;
 }
 
+// U0: the dual part of the inner step's divergence.  In the inner steps
+// (admm.cu:85-97) h2 = rho*(grad fm - psi) + mu = rho*grad fm + chi with
+// chi = mu - rho*psi fixed for the whole outer iteration, and div is linear, so
+//   dchi[z][i][y][x] = div(chi) = ks (chi_x[x'-1] - chi_x[x']) + ks (chi_y[y'-1] - chi_y[y'])
+//                    + kt (chi_t[i'-1] - chi_t[i']) + ks (chi_z[z'-1] - chi_z[z'])
+// at the dual position (z', i', y', x') = (zl+1, i+1, y+1, x+1) of the voxel
+// (div, kernels_operators.cuh:46-73) is computed once per outer iteration, after
+// the rho rule.  Thread per (chunk, y, x) column; the t and z neighbours are the
+// previous frame's / slice's own values and are carried.
+template <int MM>
+__global__ void __launch_bounds__(256)
+k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *__restrict__ dchi, TvArgs a) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int c = a.chunk0 + blockIdx.z;
+    const int n = a.n, M = a.M, nzp = a.nzp;
+    if (x >= n || y >= n) return;
+    const size_t plane = (size_t)n * n, S = (size_t)M * plane;
+    const size_t col = (size_t)y * n + x;
+    const float r = a.rho;
+    auto chi = [&](size_t d) {
+        const float4 p = psi[d], m = mu[d];
+        return make_float4(m.x - r * p.x, m.y - r * p.y, m.z - r * p.z, m.w - r * p.w);
+    };
+    float czc[MM];  // chi_z of the previous slice plane (ghost plane z' = 0 first)
+#pragma unroll
+    for (int i = 0; i < MM; i++)
+        if (i < M) czc[i] = chi(dual_idx(a, c, 0, i + 1, y + 1, x + 1)).w;
+    for (int zl = 0; zl < nzp; zl++) {
+        float ctc = chi(dual_idx(a, c, zl + 1, 0, y + 1, x + 1)).z;  // ghost frame i' = 0
+        float *o = dchi + ((size_t)c * nzp + zl) * S + col;
+#pragma unroll
+        for (int i = 0; i < MM; i++) {
+            if (i < M) {
+                const size_t d = dual_idx(a, c, zl + 1, i + 1, y + 1, x + 1);
+                const float4 c0 = chi(d);
+                const float cxm = mu[d - 1].x - r * psi[d - 1].x;
+                const float cym = mu[d - (n + 1)].y - r * psi[d - (n + 1)].y;
+                o[(size_t)i * plane] = a.ks * (cxm - c0.x) + a.ks * (cym - c0.y) + a.kt * (ctc - c0.z) +
+                                       a.ks * (czc[i] - c0.w);
+                ctc = c0.z;
+                czc[i] = c0.w;
+            }
+        }
+    }
+}
+
 // U: one inner ADMM step of the TV part fused with the R^* update (admm.cu:85-97):
-//   out = fm - 0.5*div(rho*(grad fm - psi) + mu) - 0.5*(R^* h1)_i
+//   out = fm - 0.5*(rho*div(grad fm) + dchi) - 0.5*(R^* h1)_i
 // with (R^* h1)_i = Re sum_j v_j phi_j[t_i] read directly from the frame pairs v.
 // Thread per (chunk, y, x) column marching over the chunk's slices and frames.
 // Halo slabs (chunk neighbours) come from A, the previous outer iterate
-// (rectv.cu:170-171; clamped borders kernels_operators.cuh:4-20).  The dual
-// values needed by the backward differences in z and t are the previous
-// slice's / frame's own values, so they are carried instead of reloaded.
+// (rectv.cu:170-171; clamped borders kernels_operators.cuh:4-20).  The
+// gradient values needed by the backward differences in z and t are the
+// previous slice's / frame's own values, so they are carried.
 template <int MM>
 __global__ void __launch_bounds__(256)
 k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__restrict__ out,
-         const float4 *__restrict__ psi, const float4 *__restrict__ mu, const float2 *__restrict__ v,
-         TvArgs a, int zv0) {
+         const float *__restrict__ dchi, const float2 *__restrict__ v, TvArgs a, int zv0) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int c = a.chunk0 + blockIdx.z;
     const int n = a.n, M = a.M, nzp = a.nzp;
     if (x >= n || y >= n) return;
     const bool flgl = c != 0, flgr = c != a.nchunk - 1;
-    const size_t plane = (size_t)n * n, S = (size_t)M * plane;
+    const unsigned pb = (unsigned)n * n * 4;  // bytes per frame plane (< 2^32 per slice)
+    const size_t S = (size_t)M * n * n;
     const size_t zb = (size_t)c * nzp;
-    const size_t col = (size_t)y * n + x;
-    const int xm = x > 0 ? x - 1 : x, xp = x < n - 1 ? x + 1 : x;
-    const int ym = y > 0 ? y - 1 : y, yp = y < n - 1 ? y + 1 : y;
+    // per-lane byte offsets within a frame plane; slice bases go into buffer
+    // descriptors and frame offsets into the uniform offset
+    const unsigned oc = 4u * (y * n + x);
+    const unsigned oxp = 4u * (y * n + (x < n - 1 ? x + 1 : x)), oxm = 4u * (y * n + (x > 0 ? x - 1 : x));
+    const unsigned oyp = 4u * ((y < n - 1 ? y + 1 : y) * n + x), oym = 4u * ((y > 0 ? y - 1 : y) * n + x);
     const float r = a.rho;
-    float fz[MM], fzp[MM], hzc[MM], rp[MM];
-    // own column at slice 0 and the z-gradient dual values on plane 0
+    const float rgs = r * a.gs, rgt = r * a.gt;
+    float fz[MM], fzp[MM], hzc[MM];
+    // own column at slice 0 and the z-gradient values on plane 0
+    {
+        const auto rf = buf_rsrc(fm + zb * S);
+        const auto ra = buf_rsrc(flgl ? A + (zb - 1) * S : fm + zb * S);
 #pragma unroll
-    for (int i = 0; i < MM; i++) {
-        if (i < M) {
-            fz[i] = fm[zb * S + i * plane + col];
-            const float fzm = flgl ? A[(zb - 1) * S + i * plane + col] : fz[i];
-            const size_t d = dual_idx(a, c, 0, i + 1, y + 1, x + 1);
-            hzc[i] = r * (a.gs * (fz[i] - fzm) - psi[d].w) + mu[d].w;
+        for (int i = 0; i < MM; i++) {
+            if (i < M) {
+                fz[i] = buf_ld(rf, oc, i * pb);
+                hzc[i] = rgs * (fz[i] - buf_ld(ra, oc, i * pb));
+            }
         }
     }
     for (int zl = 0; zl < nzp; zl++) {
         const size_t zg = zb + zl;
+        {
+            const auto rn = buf_rsrc(zl + 1 < nzp ? fm + (zg + 1) * S : (flgr ? A + (zg + 1) * S : fm + zg * S));
+#pragma unroll
+            for (int i = 0; i < MM; i++)
+                if (i < M) fzp[i] = buf_ld(rn, oc, i * pb);
+        }
+        const auto rf = buf_rsrc(fm + zg * S);
+        const auto rd = buf_rsrc(dchi + zg * S);
+        const auto ro = buf_rsrc(out + zg * S);
+        const auto rv = buf_rsrc(v ? (const void *)(v + (size_t)(zg - zv0) * ((M + 1) / 2) * n * n) : (const void *)dchi);
+        // t-gradient value on frame plane 0: E(t'=1) - E(t'=0) = 0
+        float htc = 0.f;
+        float2 vp = make_float2(0.f, 0.f);
 #pragma unroll
         for (int i = 0; i < MM; i++) {
             if (i < M) {
-                if (zl + 1 < nzp) fzp[i] = fm[(zg + 1) * S + i * plane + col];
-                else fzp[i] = flgr ? A[(zg + 1) * S + i * plane + col] : fz[i];
-            }
-        }
-        // R^* part: frame i of the adjoint pairs
-#pragma unroll
-        for (int i = 0; i < MM; i++) rp[i] = 0.f;
-        if (v) {
-            const float2 *vz = v + ((size_t)(zg - zv0) * ((M + 1) / 2)) * plane + col;
-#pragma unroll
-            for (int pp = 0; pp < MM / 2; pp++) {
-                if (2 * pp < M) {
-                    const float2 vp = vz[(size_t)pp * plane];
-                    rp[2 * pp] = vp.x;
-                    rp[2 * pp + 1] = vp.y;
-                }
-            }
-        }
-        // t-gradient dual value on frame plane 0: E(t'=1) - E(t'=0) = 0
-        const size_t d0 = dual_idx(a, c, zl + 1, 0, y + 1, x + 1);
-        float htc = r * (0.f - psi[d0].z) + mu[d0].z;
-        const float *fl = fm + zg * S;
-#pragma unroll
-        for (int i = 0; i < MM; i++) {
-            if (i < M) {
+                // R^* part: frame i of the adjoint pairs (re: even i, im: odd i)
+                if ((i & 1) == 0 && v) vp = buf_ld2(rv, 2 * oc, (i >> 1) * 2 * pb);
+                const float rpi = (i & 1) ? vp.y : vp.x;
                 const float f0 = fz[i];
-                const float *fi = fl + i * plane;
-                const float fxp = fi[(size_t)y * n + xp], fxm = fi[(size_t)y * n + xm];
-                const float fyp = fi[(size_t)yp * n + x], fym = fi[(size_t)ym * n + x];
+                const unsigned so = i * pb;
+                const float fxp = buf_ld(rf, oxp, so), fxm = buf_ld(rf, oxm, so);
+                const float fyp = buf_ld(rf, oyp, so), fym = buf_ld(rf, oym, so);
                 // register indices stay compile-time constants (i is unrolled)
                 const float ftp = (i + 1 < M) ? fz[(i + 1 < MM) ? i + 1 : i] : f0;
-                const size_t d = dual_idx(a, c, zl + 1, i + 1, y + 1, x + 1);
-                const float4 ps = psi[d], ms = mu[d];
-                const float px = psi[d - 1].x, mx = mu[d - 1].x;
-                const float py = psi[d - (n + 1)].y, my = mu[d - (n + 1)].y;
-                const float hx0 = r * (a.gs * (fxp - f0) - ps.x) + ms.x;
-                const float hy0 = r * (a.gs * (fyp - f0) - ps.y) + ms.y;
-                const float ht0 = r * (a.gt * (ftp - f0) - ps.z) + ms.z;
-                const float hz0 = r * (a.gs * (fzp[i] - f0) - ps.w) + ms.w;
-                const float hxm = r * (a.gs * (f0 - fxm) - px) + mx;
-                const float hym = r * (a.gs * (f0 - fym) - py) + my;
-                const float dv = a.ks * (hxm - hx0) + a.ks * (hym - hy0) + a.kt * (htc - ht0) + a.ks * (hzc[i] - hz0);
-                out[zg * S + i * plane + col] = f0 - 0.5f * dv - 0.5f * rp[i];
+                const float hx0 = rgs * (fxp - f0), hxm = rgs * (f0 - fxm);
+                const float hy0 = rgs * (fyp - f0), hym = rgs * (f0 - fym);
+                const float ht0 = rgt * (ftp - f0);
+                const float hz0 = rgs * (fzp[i] - f0);
+                const float dv = a.ks * (hxm - hx0) + a.ks * (hym - hy0) + a.kt * (htc - ht0) + a.ks * (hzc[i] - hz0) +
+                                 buf_ld(rd, oc, so);
+                buf_st(f0 - 0.5f * dv - 0.5f * rpi, ro, oc, so);
                 htc = ht0;
                 hzc[i] = hz0;
             }

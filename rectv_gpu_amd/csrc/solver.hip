@@ -384,6 +384,7 @@ struct Solver {
     DBuf<float> d_f[3];
     int ia = 0;
     DBuf<float4> d_psi, d_mu;
+    DBuf<float> d_dchi;  // div(mu - rho psi) for the current outer iteration
     DBuf<float> d_g;
     float rho = 0.5f;
     // scratch (one batch of cb chunks = zb slices)
@@ -452,6 +453,7 @@ struct Solver {
         const size_t V4 = (size_t)nch * (nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1);
         d_psi.alloc(V4, bytes);
         d_mu.alloc(V4, bytes);
+        d_dchi.alloc(V, bytes);
         d_g.alloc((size_t)nz * nt * n, bytes);
         // scratch plan
         const size_t per_slice = (size_t)MP * n * N * 8 + (size_t)MP * N * N * 8 + (size_t)MP * n * n * 8 +
@@ -608,6 +610,7 @@ struct Solver {
         for (auto &b : d_f) RTV_HIP(hipMemsetAsync(b.p, 0, V * sizeof(float), st));
         RTV_HIP(hipMemsetAsync(d_psi.p, 0, d_psi.n * sizeof(float4), st));
         RTV_HIP(hipMemsetAsync(d_mu.p, 0, d_mu.n * sizeof(float4), st));
+        RTV_HIP(hipMemsetAsync(d_dchi.p, 0, d_dchi.n * sizeof(float), st));  // psi = mu = 0
         ia = 0;
         rho = 0.5f;
         RTV_HIP(hipStreamSynchronize(st));
@@ -735,8 +738,7 @@ struct Solver {
     void launch_update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
         dim3 grid((n + 63) / 64, (n + 3) / 4, ncb);
         launch("update", [&] {
-            hipLaunchKernelGGL(k_update<MM>, grid, dim3(256), 0, st, cur, A, nxt, d_psi.p, d_mu.p, d_v.p, ta,
-                               zv0);
+            hipLaunchKernelGGL(k_update<MM>, grid, dim3(256), 0, st, cur, A, nxt, d_dchi.p, d_v.p, ta, zv0);
         });
     }
     void update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
@@ -753,6 +755,17 @@ struct Solver {
         });
         const int cnt = grid.x * grid.y * grid.z;
         launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, 3); });
+    }
+    template <int MM>
+    void launch_divchi(const TvArgs &ta) {
+        dim3 grid((n + 63) / 64, (n + 3) / 4, nch);
+        launch("divchi", [&] { hipLaunchKernelGGL(k_divchi<MM>, grid, dim3(256), 0, st, d_psi.p, d_mu.p, d_dchi.p, ta); });
+    }
+    void divchi(const TvArgs &ta) {
+        if (M <= 4) launch_divchi<4>(ta);
+        else if (M <= 8) launch_divchi<8>(ta);
+        else if (M <= 16) launch_divchi<16>(ta);
+        else launch_divchi<32>(ta);
     }
     void prox(const float *cur, const float *A, const TvArgs &ta) {
         if (M <= 4) launch_prox<4>(cur, A, ta);
@@ -804,6 +817,8 @@ struct Solver {
             else if (s > 10 * r) rho *= 0.5f;
             if (!quiet) fprintf(stderr, "iter (%d/%d), rdiff ||f(k+1)-f(k)||:%f \n", it, niter, rdiff);
             ia = cur;
+            // dual part of the next outer iteration's inner steps, with the new rho
+            divchi(tv_args(lam1, lam0));
         }
     }
 
@@ -948,6 +963,7 @@ struct Solver {
         for (auto &b : d_f) b.release(t);
         d_psi.release(t);
         d_mu.release(t);
+        d_dchi.release(t);
         d_g.release(t);
         d_H.release(t);
         d_G.release(t);
