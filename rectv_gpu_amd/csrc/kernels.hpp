@@ -199,15 +199,16 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
     }
 }
 
-// Gaussian window weights exp(-beta (a - K - frac)^2), a = 0..8 (the separable
-// factors of kernels_radonusfft.cuh:114-118; the PI/mu factor is folded into
-// the ray tables).
+// One separable Gaussian window factor exp(-beta d^2) (kernels_radonusfft.cuh:114-118;
+// the PI/mu factor is folded into the ray tables).  exp through the hardware
+// exp2: relative error a few 1e-7 for the taps that matter (beta d^2 < 10); the
+// forward and adjoint use the same function, so they stay exact transposes.
+__device__ __forceinline__ float gauss(float beta, float d) { return __expf(-beta * d * d); }
+
+// Gaussian window weights exp(-beta (a - K - frac)^2), a = 0..8.
 __device__ __forceinline__ void window_weights(float frac, float beta, float *w) {
 #pragma unroll
-    for (int a = 0; a < KW; a++) {
-        const float d = (float)(a - 4) - frac;
-        w[a] = expf(-beta * d * d);
-    }
+    for (int a = 0; a < KW; a++) w[a] = gauss(beta, (float)(a - 4) - frac);
 }
 
 // F3: tile-binned Gaussian gather from the MP pair spectra of slice z (replaces
@@ -471,8 +472,8 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
             const int r = part * (TH / 4) + h;
             const int ay = r - ly0, ax = r - lx0;
             const float dyv = (float)(ay - 4) - g.fy, dxv = (float)(ax - 4) - g.fx;
-            wyr[it][r] = ((unsigned)ay < (unsigned)KW) ? expf(-beta * dyv * dyv) : 0.f;
-            wxr[it][r] = ((unsigned)ax < (unsigned)KW) ? expf(-beta * dxv * dxv) : 0.f;
+            wyr[it][r] = ((unsigned)ay < (unsigned)KW) ? gauss(beta, dyv) : 0.f;
+            wxr[it][r] = ((unsigned)ax < (unsigned)KW) ? gauss(beta, dxv) : 0.f;
         }
         if (tid < 4 * NCP) tile[CELLS * NCP + tid] = 0.f;
     }
@@ -726,19 +727,16 @@ constexpr int SM_ZW = 4;    // slices per workgroup
 template <int NB>           // 16-column blocks: MP <= 8*NB (NB = 1 or 2)
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))  // two workgroups per CU
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
-          const int *__restrict__ sids, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf, int n,
-          int nt, int MP, float beta, int ntiles, int nzb) {
+          const int *__restrict__ sids, const TileEntry *__restrict__ tiles, const int2 *__restrict__ arange,
+          const float2 *__restrict__ cf, int n, int nt, int MP, float beta, int ntiles, int nzb) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
     constexpr int NC = 16 * NB;
     constexpr int HALF = 8 * NB;       // column of the imaginary part of pair 0
     constexpr int NCP = NC + 1;        // padded row of P
-    constexpr int NBIN = TS + KW - 1;  // window rows ly0 in [-8, TS)
     constexpr int NPP = HALF / 8;      // pairs per staging thread
     __shared__ float wyr[SM_CH][TS];
     __shared__ float wxr[SM_CH][TS];
     __shared__ float sP[SM_ZW][SM_CH][NCP];
-    __shared__ int sly[SM_CH];
-    __shared__ int bfirst[NBIN], blast[NBIN];
     const int tid = threadIdx.x;
     const int N = 2 * n, nT = N / TS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -784,12 +782,9 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
     };
     if (te.count > 0) prefetch(0);
     for (int c0 = 0; c0 < te.count; c0 += SM_CH) {
-        const int cnt = min(SM_CH, te.count - c0);
+        // run of items covering this wave's row (host table, rounded to whole MFMA steps below)
+        const int2 run = arange[((size_t)te.pad + c0 / SM_CH) * TS + r];
         __syncthreads();
-        if (tid < NBIN) {
-            bfirst[tid] = cnt;
-            blast[tid] = 0;
-        }
         if (pid >= 0) {
             const SampleGeom g = pg;
             int dy = g.row0 - ty0;
@@ -798,15 +793,14 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
             if (dx < 0) dx += N;
             const int ly0 = dy < TS ? dy : dy - N;
             const int lx0 = dx < TS ? dx : dx - N;
-            if (part == 0) sly[sm] = ly0;
             // weight-table entries 2*part, 2*part+1 (zero outside the window)
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int i = 2 * part + h;
                 const int ay = i - ly0, ax = i - lx0;
                 const float dyv = (float)(ay - 4) - g.fy, dxv = (float)(ax - 4) - g.fx;
-                wyr[sm][i] = ((unsigned)ay < (unsigned)KW) ? expf(-beta * dyv * dyv) : 0.f;
-                wxr[sm][i] = ((unsigned)ax < (unsigned)KW) ? expf(-beta * dxv * dxv) : 0.f;
+                wyr[sm][i] = ((unsigned)ay < (unsigned)KW) ? gauss(beta, dyv) : 0.f;
+                wxr[sm][i] = ((unsigned)ax < (unsigned)KW) ? gauss(beta, dxv) : 0.f;
             }
 #pragma unroll
             for (int zz = 0; zz < SM_ZW; zz++) {
@@ -835,23 +829,9 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
         }
         if (c0 + SM_CH < te.count) prefetch(c0 + SM_CH);
         __syncthreads();
-        // first/last index of each window-row bin (the list is sorted by ly0)
-        if (tid < cnt) {
-            const int v = sly[tid];
-            if (tid == 0 || sly[tid - 1] != v) bfirst[v + KW - 1] = tid;
-            if (tid == cnt - 1 || sly[tid + 1] != v) blast[v + KW - 1] = tid + 1;
-        }
-        __syncthreads();
-        // items covering row r: bins ly0 = r-8 .. r; the run is widened to
-        // multiples of 4 (slots stay < SM_CH)
-        int lo = cnt, hi = 0;
-#pragma unroll
-        for (int q = 0; q < KW; q++) {
-            lo = min(lo, bfirst[r + q]);
-            hi = max(hi, blast[r + q]);
-        }
-        lo &= ~3;
-        hi = (hi + 3) & ~3;
+        // the run widened to multiples of 4 (slots stay < SM_CH); empty: lo > hi
+        const int lo = __builtin_amdgcn_readfirstlane(run.x) & ~3;
+        const int hi = (__builtin_amdgcn_readfirstlane(run.y) + 3) & ~3;
         const float *pw = &wyr[lk][r];
         const float *px = &wxr[lk][li];
         const float *pp = &sP[0][lk][li];

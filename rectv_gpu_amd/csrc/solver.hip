@@ -63,7 +63,8 @@ struct HostGeom {
     std::vector<int4> fgroups;
     std::vector<int> fitems;  // forward items per origin tile for the matrix-core gather (-1 = padding)
     std::vector<int> asids;
-    std::vector<TileEntry> ftiles, fmtiles, atiles;
+    std::vector<TileEntry> ftiles, fmtiles, atiles;  // atiles[].pad: first chunk of the tile
+    std::vector<int2> arange;  // per (adjoint chunk, tile row): [lo, hi) of the items covering the row
     std::vector<float2> pre_f, post_a, cf;
     std::vector<float> post_f, pre_a, deap;
     float beta = 0.f;
@@ -275,6 +276,31 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
         hg.atiles.resize(ntile);
         for (int t = 0; t < ntile; t++) hg.atiles[t] = TileEntry{t, off[t], cnt[t], 0};
+        // per chunk of SM_CH items and tile row r: the items whose window rows
+        // ly0 .. ly0+8 contain r form one run (items are sorted by ly0)
+        hg.arange.clear();
+        for (int t = 0; t < ntile; t++) {
+            TileEntry &te = hg.atiles[t];
+            te.pad = (int)(hg.arange.size() / TS);
+            const int ty0 = (t / nT) * TS;
+            for (int c0 = 0; c0 < te.count; c0 += SM_CH) {
+                const int cn = std::min(SM_CH, te.count - c0);
+                std::vector<int> ly(cn);
+                for (int i = 0; i < cn; i++) {
+                    const int dy = (hg.geo[hg.asids[te.start + c0 + i]].row0 - ty0 + N) % N;
+                    ly[i] = dy < TS ? dy : dy - N;
+                }
+                for (int r = 0; r < TS; r++) {
+                    int lo = cn, hi = 0;
+                    for (int i = 0; i < cn; i++)
+                        if (ly[i] <= r && r <= ly[i] + KW - 1) {
+                            lo = std::min(lo, i);
+                            hi = std::max(hi, i + 1);
+                        }
+                    hg.arange.push_back(make_int2(lo, hi));
+                }
+            }
+        }
         if (std::getenv("RECTV_DENSE_FIRST"))
             std::stable_sort(hg.atiles.begin(), hg.atiles.end(),
                              [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
@@ -399,6 +425,7 @@ struct Solver {
     DBuf<int> d_fitems;
     DBuf<int> d_asids;
     DBuf<TileEntry> d_ftiles, d_fmtiles, d_atiles;
+    DBuf<int2> d_arange;
     DBuf<float2> d_cf, d_pre_f, d_post_a;
     DBuf<float> d_post_f, d_pre_a, d_deap;
     int nf_entries = 0, nfm_entries = 0, na_tiles = 0, fgs = 4;
@@ -592,6 +619,7 @@ struct Solver {
         gather_valu = std::getenv("RECTV_GATHER") && std::string(std::getenv("RECTV_GATHER")) == "valu";
         upload(d_asids, hg.asids, bytes, st);
         upload(d_atiles, hg.atiles, bytes, st);
+        upload(d_arange, hg.arange, bytes, st);
         upload(d_cf, hg.cf, bytes, st);
         upload(d_pre_f, hg.pre_f, bytes, st);
         upload(d_post_f, hg.post_f, bytes, st);
@@ -690,10 +718,10 @@ struct Solver {
                 // MP <= 16 (M <= 32): one or two 16-column blocks
                 if (MP <= 8)
                     hipLaunchKernelGGL(k_scatter<1>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
-                                       d_atiles.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
+                                       d_atiles.p, d_arange.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
                 else
                     hipLaunchKernelGGL(k_scatter<2>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
-                                       d_atiles.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
+                                       d_atiles.p, d_arange.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
             });
         }
         {
@@ -979,6 +1007,7 @@ struct Solver {
         d_fitems.release(t);
         d_fmtiles.release(t);
         d_atiles.release(t);
+        d_arange.release(t);
         d_cf.release(t);
         d_pre_f.release(t);
         d_post_a.release(t);
