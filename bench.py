@@ -115,6 +115,88 @@ def cpu_baseline(w, titer):
                       % (n, nt, m, t_inner, t_prox, titer)}
 
 
+def batched_bench(count, niter, titer, workers=8):
+    """Batched-instances mix: `count` uneven jobs solved one after another with the
+    plain API (Solver.recon each) and then together (batch.solve_batch); same
+    inputs, same arithmetic.  Value = sum of n^2*Nz*M*niter over jobs / wall time,
+    host set-up (geometry tables, uploads) included in both."""
+    from rectv_gpu_amd import synthetic, batch
+    specs = synthetic.batch_specs(count)
+    mk = lambda: [synthetic.make_job(sp, niter, titer) for sp in specs]
+    jobs_a = mk()
+    vox = sum(j.voxels * j.niter for j in jobs_a)
+    print("bench: batched mix of %d jobs generated (%.2f G voxel-iterations)" % (count, vox / 1e9),
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batch.solve_sequential(jobs_a)
+    t_seq = time.perf_counter() - t0
+    print("bench: sequential plain solves %.2f s" % t_seq, file=sys.stderr, flush=True)
+    ref = [j.result for j in jobs_a]
+    del jobs_a
+    jobs_b = mk()
+    t0 = time.perf_counter()
+    batch.solve_batch(jobs_b, workers=workers)
+    t_bat = time.perf_counter() - t0
+    print("bench: batched %.2f s" % t_bat, file=sys.stderr, flush=True)
+    diff = max(float(np.linalg.norm(j.result.astype(np.float64) - r) / max(np.linalg.norm(r), 1e-30))
+               for j, r in zip(jobs_b, ref))
+    return {"workload": "%d jobs: N in {128,256,384,512}, Nz U{4..32}, Nrot {2,4,8} x {64,100,128} "
+                        "projections, M {4,8,16}, Nzp=Nz, niter=%d, titer=%d" % (count, niter, titer),
+            "value": round(vox / t_bat, 1), "unit": "voxel-iter/s", "sequential_value": round(vox / t_seq, 1),
+            "speedup_vs_sequential": round(t_seq / t_bat, 3), "batched_s": round(t_bat, 3),
+            "sequential_s": round(t_seq, 3), "voxel_iterations": vox, "max_rel_l2_vs_plain": diff}
+
+
+def served_bench(count, niter, titer, rate, seed=7, max_active=16, sample_frac=0.1):
+    """Serving mix: `count` jobs (the batched mix plus 5% N=1024, Nz=32, M=16 jobs)
+    arriving as a Poisson process of `rate` jobs/s, admitted by device footprint,
+    iterations interleaved.  Reports throughput over the whole run, p50/p99 job
+    latency (arrival to result on the host), and, on a sample of the jobs, the
+    relative L2 difference of the served result from a plain solve of that job."""
+    import threading
+    from rectv_gpu_amd import synthetic, batch
+    specs = synthetic.serve_specs(count)
+    jobs = [synthetic.make_job(sp, niter, titer) for sp in specs]
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, count)
+    print("bench: serving mix of %d jobs generated" % count, file=sys.stderr, flush=True)
+    srv = batch.Server(max_active=max_active, ahead=2, workers=4)
+
+    def producer():
+        t0 = time.perf_counter()
+        at = np.cumsum(gaps)
+        for j, a in zip(jobs, at):
+            d = t0 + a - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            srv.submit(j)
+
+    th = threading.Thread(target=producer)
+    t0 = time.perf_counter()
+    th.start()
+    try:
+        srv.run(until=lambda: not th.is_alive() and srv.pending() == 0)
+    finally:
+        th.join()
+        srv.close()
+    t1 = time.perf_counter()
+    errors = [j.error for j in jobs if j.error]
+    vox = sum(j.voxels * j.iterations for j in jobs if not j.error)
+    lat = batch.latency_stats(jobs)
+    print("bench: served %d jobs in %.2f s, p50 %.2f s, p99 %.2f s" % (count, t1 - t0, lat["p50_s"], lat["p99_s"]),
+          file=sys.stderr, flush=True)
+    k = max(1, int(round(sample_frac * count)))
+    sample = sorted(range(count), key=lambda i: jobs[i].voxels)[:k]
+    diff = 0.0
+    for i in sample:
+        r = batch.solve_sequential([synthetic.make_job(specs[i], niter, titer)])[0].result
+        diff = max(diff, float(np.linalg.norm(jobs[i].result.astype(np.float64) - r) / max(np.linalg.norm(r), 1e-30)))
+    return {"workload": "%d jobs (batched mix + 5%% N=1024 Nz=32 M=16), Poisson arrivals %.0f jobs/s, niter=%d, "
+                        "titer=%d, max %d active" % (count, rate, niter, titer, max_active),
+            "value": round(vox / (t1 - t0), 1), "unit": "voxel-iter/s", "wall_s": round(t1 - t0, 3),
+            "latency": lat, "jobs": count, "errors": len(errors), "peak_active": srv.peak_active,
+            "sample_max_rel_l2_vs_plain": diff, "sample_jobs": k}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +207,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true", help="also run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--mode", default="all", choices=["all", "plain", "batched", "served"],
+                    help="plain: the headline solve only; batched / served: those forms only; all: each")
+    ap.add_argument("--jobs", type=int, default=64, help="jobs in the batched / served mixes")
+    ap.add_argument("--job-niter", type=int, default=16, help="outer iterations per batched / served job")
+    ap.add_argument("--job-titer", type=int, default=4)
+    ap.add_argument("--rate", type=float, default=20.0, help="serving arrival rate (jobs/s)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build drives exactly one GPU; --gpus %d is not supported "
@@ -132,6 +220,14 @@ def main():
         sys.exit(2)
     os.environ.setdefault("RECTV_QUIET", "1")
     from rectv_gpu_amd import Solver, synthetic
+    if a.mode in ("batched", "served"):
+        out = {"metric": METRIC, "n_gpus": 1, "dtype": "f32", "data": "synthetic"}
+        if a.mode == "batched":
+            out["batched"] = batched_bench(a.jobs, a.job_niter, a.job_titer)
+        else:
+            out["served"] = served_bench(a.jobs, a.job_niter, a.job_titer, a.rate)
+        print(json.dumps(out), flush=True)
+        return
 
     t_setup = time.perf_counter()
     w = synthetic.workload(a.config, seed=a.seed)
@@ -195,6 +291,9 @@ def main():
     }
     if a.copy_probe:
         copy_probe()
+    if a.mode == "all":
+        out["batched"] = batched_bench(a.jobs, a.job_niter, a.job_titer)
+        out["served"] = served_bench(a.jobs, a.job_niter, a.job_titer, a.rate)
     if not a.no_cpu:
         print("bench: CPU baseline sample ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, a.titer)

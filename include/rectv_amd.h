@@ -92,6 +92,33 @@ int rtv_get_profile(rtv_solver *h, char *buf, int len);
 /* Device bytes held by the solver (persistent state + scratch). */
 long long rtv_device_bytes(rtv_solver *h);
 
+/* ---- asynchronous, graph-replayed and batched forms (serving / batching).  No
+ * reference counterpart: the reference runs one blocking solve per process
+ * (rectv::run, rectv.cu:102-252) and keeps rho on the host (rectv.cu:125, 220-232);
+ * here rho, its rule and the per-iteration norms stay on the device, so outer
+ * iterations can be enqueued without host round trips. */
+/* Enqueue niter outer iterations on the solver's stream and return at once. */
+int rtv_iterate_async(rtv_solver *h, int niter, int titer, float lambda0, float lambda1);
+/* *idle = 1 when all enqueued work of the solver has finished, else 0. */
+int rtv_query(rtv_solver *h, int *idle);
+/* Outer iterations enqueued since the last rtv_reset. */
+int rtv_iterations(rtv_solver *h, long long *count);
+/* Rows [first, first+count) since reset (waits for the solver's stream), 5 doubles
+ * each: r, s, rho used, ||f(k+1)-f(k)||^2, ||f(k+1)||^2.  The last 1024 rows are held. */
+int rtv_history(rtv_solver *h, long long first, int count, double *rows);
+/* on != 0: each outer iteration is captured once as a HIP graph and replayed. */
+int rtv_set_graph(rtv_solver *h, int on);
+/* Times the launch-shape candidates of the gather and scatter on this solver's
+ * geometry and keeps the fastest (cached per shape in the process).  Results are
+ * bitwise independent of the choice.  Outputs (may be NULL): slices per workgroup. */
+int rtv_autotune(rtv_solver *h, int *gather_slices, int *scatter_slices);
+/* Runs niter[j] outer iterations of every solver hs[j] (distinct, inputs set),
+ * interleaved round robin on their own streams; returns when all have finished. */
+int rtv_batch_iterate(rtv_solver **hs, int count, const int *niter, const int *titer,
+                      const float *lambda0, const float *lambda1);
+/* Device bytes a solver of these sizes holds (upper bound), for admission control. */
+int rtv_footprint(int n, int ntheta, int m, int nz, int nzp, long long *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,14 @@ SIGNATURES = {
     "rtv_set_profiling": (_I, [_P, _I]),
     "rtv_get_profile": (_I, [_P, ctypes.c_char_p, _I]),
     "rtv_device_bytes": (ctypes.c_longlong, [_P]),
+    "rtv_iterate_async": (_I, [_P, _I, _I, _F, _F]),
+    "rtv_query": (_I, [_P, ctypes.POINTER(_I)]),
+    "rtv_iterations": (_I, [_P, ctypes.POINTER(ctypes.c_longlong)]),
+    "rtv_history": (_I, [_P, ctypes.c_longlong, _I, _P]),
+    "rtv_set_graph": (_I, [_P, _I]),
+    "rtv_autotune": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "rtv_batch_iterate": (_I, [ctypes.POINTER(_P), _I, _P, _P, _P, _P]),
+    "rtv_footprint": (_I, [_I, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 

@@ -391,12 +391,12 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
 // conj(beta_p[k]) mirror) and two cross-lane steps finish the sum over pairs.
 constexpr int GM_ITEMS = 128;  // items per workgroup: 8 waves x 16
 constexpr int GM_KS = 4;       // max steps of 4 cells per block (the host packs blocks so)
-constexpr int GM_ZG = 8;       // slices per workgroup
+constexpr int GM_ZG = 8;       // default slices per workgroup (runtime argument zg)
 template <int NB>              // 16-column blocks: MP <= 8*NB
 __global__ void __launch_bounds__(512)
 k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
               const int *__restrict__ items, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf,
-              int n, int nt, int MP, float beta, int nentries, int nzb) {
+              int n, int nt, int MP, float beta, int nentries, int nzb, int zg) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
     constexpr int NC = 16 * NB, HALF = 8 * NB, NCP = NC + 1;
     constexpr int CELLS = TH * TH;
@@ -412,8 +412,8 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     const int N = 2 * n, nT = N / TS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int e = L % nentries;
-    const int z0 = (L / nentries) * GM_ZG;
-    const int zc = min(GM_ZG, nzb - z0);
+    const int z0 = (L / nentries) * zg;
+    const int zc = min(zg, nzb - z0);
     const TileEntry te = tiles[e];
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
     const int cnt = te.count;
@@ -706,10 +706,10 @@ constexpr int GS = 4;  // sample slots per origin group (int4 group records)
 // A1: tile-owned Gaussian scatter on the FP32 matrix cores (replaces gather ADJ
 // with global atomicAdd, kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62,
 // run once per basis function after mulphi(-1), operators.cu:47-56).
-// One workgroup (16 waves) owns one TS x TS tile of the 2n grid for SM_ZW
+// One workgroup (16 waves) owns one TS x TS tile of the 2n grid for ZW
 // slices; wave r owns tile row r.  The items (direct and mirror windows of the
 // samples, see F3) whose 9x9 window intersects the tile are listed sorted by
-// window row.  A chunk of them is staged in LDS once for the SM_ZW slices:
+// window row.  A chunk of them is staged in LDS once for the ZW slices:
 // 16-wide weight rows wyr[s][r] = wy[r-ly0], wxr[s][i] = wx[i-lx0] (zero outside
 // the window) and, per slice, P[s][col] = conj(cf[k][kind][p]) * q' (q' = q
 // direct, conj(q) mirror) with the real part in column p and the imaginary part
@@ -723,8 +723,8 @@ constexpr int GS = 4;  // sample slots per origin group (int4 group records)
 // without tests.  Lane (li, lk) ends with cells 4lk..4lk+3 of column li and
 // writes them straight to HBM.  No atomics; fixed summation order.
 constexpr int SM_CH = 128;  // items staged per chunk (8 staging threads each)
-constexpr int SM_ZW = 4;    // slices per workgroup
-template <int NB>           // 16-column blocks: MP <= 8*NB (NB = 1 or 2)
+constexpr int SM_ZW = 4;    // default slices per workgroup (template argument ZW)
+template <int NB, int ZW>   // 16-column blocks: MP <= 8*NB (NB = 1 or 2); ZW slices per workgroup
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))  // two workgroups per CU
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
           const int *__restrict__ sids, const TileEntry *__restrict__ tiles, const int2 *__restrict__ arange,
@@ -736,28 +736,28 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
     constexpr int NPP = HALF / 8;      // pairs per staging thread
     __shared__ float wyr[SM_CH][TS];
     __shared__ float wxr[SM_CH][TS];
-    __shared__ float sP[SM_ZW][SM_CH][NCP];
+    __shared__ float sP[ZW][SM_CH][NCP];
     const int tid = threadIdx.x;
     const int N = 2 * n, nT = N / TS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int e = L % ntiles;
-    const int z0 = (L / ntiles) * SM_ZW;
-    const int zc = min(SM_ZW, nzb - z0);
+    const int z0 = (L / ntiles) * ZW;
+    const int zc = min(ZW, nzb - z0);
     const TileEntry te = tiles[e];
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
     const int r = tid >> 6, l = tid & 63;
     const int li = l & 15, lk = l >> 4;
     const int sm = tid >> 3, part = tid & 7;  // staging: item slot, eighth
-    v4f acc[SM_ZW][NB];
+    v4f acc[ZW][NB];
 #pragma unroll
-    for (int zz = 0; zz < SM_ZW; zz++)
+    for (int zz = 0; zz < ZW; zz++)
 #pragma unroll
         for (int b = 0; b < NB; b++) acc[zz][b] = (v4f){0.f, 0.f, 0.f, 0.f};
     // chunk staging data are prefetched into registers one chunk ahead, so the
     // dependent global loads (item -> geometry, q, coefficients) overlap the MFMA loop
     int pid = -1;
     SampleGeom pg = {0, 0, 0.f, 0.f};
-    float2 pq[SM_ZW], pph[NPP];
+    float2 pq[ZW], pph[NPP];
     auto prefetch = [&](int c0) {
         const int cnt = min(SM_CH, te.count - c0);
         pid = -1;
@@ -767,7 +767,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
             const int sid = pid >> 1;
             const bool mir = pid & 1;
 #pragma unroll
-            for (int zz = 0; zz < SM_ZW; zz++) {
+            for (int zz = 0; zz < ZW; zz++) {
                 float2 q = zz < zc ? gq[((size_t)(z0 + zz) * nt * n + sid) * 2] : make_float2(0.f, 0.f);
                 if (mir) q.y = -q.y;
                 pq[zz] = q;
@@ -803,7 +803,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
                 wxr[sm][i] = ((unsigned)ax < (unsigned)KW) ? gauss(beta, dxv) : 0.f;
             }
 #pragma unroll
-            for (int zz = 0; zz < SM_ZW; zz++) {
+            for (int zz = 0; zz < ZW; zz++) {
 #pragma unroll
                 for (int q = 0; q < NPP; q++) {
                     const int j = part + 8 * q;
@@ -820,7 +820,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
                 wxr[sm][2 * part + h] = 0.f;
             }
 #pragma unroll
-            for (int zz = 0; zz < SM_ZW; zz++)
+            for (int zz = 0; zz < ZW; zz++)
 #pragma unroll
                 for (int q = 0; q < NPP; q++) {
                     sP[zz][sm][part + 8 * q] = 0.f;
@@ -839,7 +839,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
         for (int k0 = lo; k0 < hi; k0 += 4) {
             const float av = pw[k0 * TS] * px[k0 * TS];
 #pragma unroll
-            for (int zz = 0; zz < SM_ZW; zz++) {
+            for (int zz = 0; zz < ZW; zz++) {
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
                     const float bv = pp[(zz * SM_CH + k0) * NCP + b * 16];
@@ -852,7 +852,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
     // column p, imaginary part in column HALF + p
     float2 *Gt = G + (size_t)(ty0 + r) * N + tx0 + 4 * lk;
 #pragma unroll
-    for (int zz = 0; zz < SM_ZW; zz++) {
+    for (int zz = 0; zz < ZW; zz++) {
         float re[4], im[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -995,7 +995,8 @@ struct TvArgs {
     int n, M, nzp, nchunk;  // nchunk = total chunks (border flags)
     int chunk0;             // first chunk handled by this launch
     float gs, gt, ks, kt;
-    float rho, lam0;
+    const float *rho;  // device-resident penalty (updated by k_rho_rule between outer iterations)
+    float lam0;
 };
 
 __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i, int y, int x) {
@@ -1021,7 +1022,7 @@ k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *_
     if (x >= n || y >= n) return;
     const size_t plane = (size_t)n * n, S = (size_t)M * plane;
     const size_t col = (size_t)y * n + x;
-    const float r = a.rho;
+    const float r = *a.rho;
     auto chi = [&](size_t d) {
         const float4 p = psi[d], m = mu[d];
         return make_float4(m.x - r * p.x, m.y - r * p.y, m.z - r * p.z, m.w - r * p.w);
@@ -1075,7 +1076,7 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
     const unsigned oc = 4u * (y * n + x);
     const unsigned oxp = 4u * (y * n + (x < n - 1 ? x + 1 : x)), oxm = 4u * (y * n + (x > 0 ? x - 1 : x));
     const unsigned oyp = 4u * ((y < n - 1 ? y + 1 : y) * n + x), oym = 4u * ((y > 0 ? y - 1 : y) * n + x);
-    const float r = a.rho;
+    const float r = *a.rho;
     const float rgs = r * a.gs, rgt = r * a.gt;
     float fz[MM], fzp[MM], hzc[MM];
     // own column at slice 0 and the z-gradient values on plane 0
@@ -1136,18 +1137,21 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 
 // P1: after the inner steps: h2 = grad fm; psi = shrink(h2 + mu/rho, lam0/rho);
 // mu += rho (h2 - psi); partial sums r = |h2 - psi|^2, s = |grad(f_old - fm)|^2
-// and ||fm - f_old||^2 (admm.cu:101-117, rectv.cu:224-237).  Thread per dual
-// column (chunk, y', x') over all (n+1)^2 positions, ghost entries included.
+// and ||fm - f_old||^2 (admm.cu:101-117, rectv.cu:224-237), plus ||fm||^2 (the
+// scale of the relative change used by callers that stop early).  Thread per
+// dual column (chunk, y', x') over all (n+1)^2 positions, ghost entries included.
+constexpr int PROX_W = 4;  // partial sums per workgroup
 template <int MM>
 __global__ void __launch_bounds__(256)
 k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__restrict__ psi,
        float4 *__restrict__ mu, TvArgs a, double *__restrict__ partial) {
-    __shared__ double red[3][4];
+    __shared__ double red[PROX_W][4];
     const int xq = blockIdx.x * 64 + (threadIdx.x & 63);  // x' in [0, n]
     const int yq = blockIdx.y * 4 + (threadIdx.x >> 6);   // y' in [0, n]
     const int c = a.chunk0 + blockIdx.z;
     const int n = a.n, M = a.M, nzp = a.nzp;
-    double sr = 0.0, ss = 0.0, sc = 0.0;
+    const float r = *a.rho;
+    double sr = 0.0, ss = 0.0, sc = 0.0, sf = 0.0;
     if (xq <= n && yq <= n) {
         const bool flgl = c != 0, flgr = c != a.nchunk - 1;
         const size_t plane = (size_t)n * n, S = (size_t)M * plane;
@@ -1210,10 +1214,12 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
                     const float dx = a.gs * (xDv - d0), dy = a.gs * (yDv - d0);
                     const float dt = a.gt * ((iq < M ? cD[it] : d0) - d0), dz = a.gs * (nD[ic] - d0);
                     ss += (double)dx * dx + (double)dy * dy + (double)dt * dt + (double)dz * dz;
-                    if (zq >= 1 && iq >= 1 && yq >= 1 && xq >= 1) sc += (double)d0 * d0;
+                    if (zq >= 1 && iq >= 1 && yq >= 1 && xq >= 1) {
+                        sc += (double)d0 * d0;
+                        sf += (double)e0 * e0;
+                    }
                     const size_t d = dual_idx(a, c, zq, iq, yq, xq);
                     float4 m4 = mu[d];
-                    const float r = a.rho;
                     float4 p4 = make_float4(hx + m4.x / r, hy + m4.y / r, ht + m4.z / r, hz + m4.w / r);
                     const float za = sqrtf(p4.x * p4.x + p4.y * p4.y + p4.z * p4.z + p4.w * p4.w);
                     const float th = a.lam0 / r;
@@ -1248,18 +1254,45 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
         sr += __shfl_down(sr, o, 64);
         ss += __shfl_down(ss, o, 64);
         sc += __shfl_down(sc, o, 64);
+        sf += __shfl_down(sf, o, 64);
     }
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red[0][w] = sr;
         red[1][w] = ss;
         red[2][w] = sc;
+        red[3][w] = sf;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        for (int q = 0; q < 3; q++) partial[3 * b + q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
+        for (int q = 0; q < PROX_W; q++) partial[PROX_W * b + q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
     }
+}
+
+// The penalty rule between outer iterations (rectv.cu:220-232), on the device so
+// that an outer iteration needs no host round trip (and can be replayed as one
+// graph): with red = {sum |h2 - psi|^2, sum |grad(f_old - fm)|^2, ||fn - f||^2,
+// ||fn||^2}, r = red[0], s = rho^2 red[1]; rho *= 2 if r > 10 s, rho /= 2 if
+// s > 10 r.  Row *iter of hist (capacity cap rows, wrapping) receives
+// {r, s, rho used, ||fn - f||^2, ||fn||^2}; *iter is then incremented.
+__global__ void k_rho_rule(const double *__restrict__ red, float *__restrict__ rho, double *__restrict__ hist,
+                           int *__restrict__ iter, int cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const float rh = *rho;
+    const double r = red[0], s = (double)rh * rh * red[1];
+    const int it = *iter;
+    double *h = hist + (size_t)(it % cap) * 5;
+    h[0] = r;
+    h[1] = s;
+    h[2] = rh;
+    h[3] = red[2];
+    h[4] = red[3];
+    float nr = rh;
+    if (r > 10 * s) nr = rh * 2;
+    else if (s > 10 * r) nr = rh * 0.5f;
+    *rho = nr;
+    *iter = it + 1;
 }
 
 // Deterministic final reduction of `count` groups of `width` doubles (fixed order).

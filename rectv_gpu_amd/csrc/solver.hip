@@ -9,6 +9,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <tuple>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -412,7 +414,20 @@ struct Solver {
     DBuf<float4> d_psi, d_mu;
     DBuf<float> d_dchi;  // div(mu - rho psi) for the current outer iteration
     DBuf<float> d_g;
-    float rho = 0.5f;
+    // penalty rho, its rule and the per-iteration norms live on the device
+    // (k_rho_rule), so an outer iteration needs no host round trip
+    DBuf<float> d_rho;
+    DBuf<double> d_hist;  // [HIST_CAP][5] ring: r, s, rho used, ||fn - f||^2, ||fn||^2
+    DBuf<int> d_iter;     // rows written since reset
+    static constexpr int HIST_CAP = 1024;
+    long long iters_done = 0;  // outer iterations enqueued since reset
+    // one captured outer iteration per rotation state of the three f buffers
+    bool use_graph = false;
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+    int gk_titer[3] = {0, 0, 0};
+    float gk_lam0[3] = {0.f, 0.f, 0.f}, gk_lam1[3] = {0.f, 0.f, 0.f};
+    // launch shapes chosen by autotune() (results are bitwise independent of them)
+    int gather_zg = GM_ZG, scatter_zw = SM_ZW;
     // scratch (one batch of cb chunks = zb slices)
     int cb = 1, zb = 1;
     DBuf<float2> d_H, d_G, d_v, d_gq;
@@ -457,6 +472,22 @@ struct Solver {
         }
     }
 
+    // Device bytes of a solver whose slice batch holds all chunks (what init_device
+    // allocates when that fits), with an upper bound for the geometry tables.
+    static size_t plan_bytes(int n, int nt, int M, int nz, int nzp) {
+        const size_t N = 2 * (size_t)n, MP = (M + 1) / 2, nch = nz / nzp;
+        const size_t V = (size_t)nz * M * n * n;
+        const size_t V4 = nch * (nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1);
+        const size_t state = 3 * V * 4 + 2 * V4 * 16 + V * 4 + (size_t)nz * nt * n * 4;
+        const size_t per_slice = MP * n * N * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 16 + (size_t)nt * n * 4;
+        const size_t ni = 2 * (size_t)nt * n, ntile = (N / TS) * (N / TS);
+        // geo 16 B, VALU groups <= 16 B, gather slots <= 16 ints, adjoint lists <= 4 ints per item,
+        // row runs 16 int2 per tile and chunk; tables per ray / pixel
+        const size_t geom = ni * (16 + 16 + 64 + 16) + (ntile + 4 * ni / SM_CH + 1) * TS * 8 + 3 * ntile * 16 +
+                            (size_t)nt * 2 * MP * 8 + 8 * (size_t)n * 8 + 2 * (N + n) * 8;
+        return state + per_slice * nz + geom + (1 << 20);
+    }
+
     void init_device() {
         N = 2 * n;
         RTV_REQUIRE(make_plan(n, plan_n) && make_plan(N, plan_N),
@@ -498,8 +529,11 @@ struct Solver {
         d_h1.alloc((size_t)zb * nt * n, bytes);
         const size_t pb = (size_t)((n + 1 + 63) / 64) * ((n + 1 + 3) / 4) * nch;
         const size_t rb = (size_t)((zb * nt + T_ray - 1) / T_ray);
-        d_part.alloc(3 * std::max(pb, rb) + 16, bytes);
+        d_part.alloc(PROX_W * std::max(pb, rb) + 16, bytes);
         d_red.alloc(8, bytes);
+        d_rho.alloc(1, bytes);
+        d_hist.alloc((size_t)HIST_CAP * 5, bytes);
+        d_iter.alloc(1, bytes);
         RTV_HIP(hipStreamSynchronize(st));
         if (std::getenv("RECTV_VERBOSE"))
             fprintf(stderr,
@@ -601,6 +635,7 @@ struct Solver {
             RTV_REQUIRE(std::isfinite(phi[i]), "phi contains non-finite values");
         for (int k = 0; k < nt; k++) RTV_REQUIRE(std::isfinite(theta[k]), "theta contains non-finite values");
         RTV_REQUIRE(std::isfinite(center), "center must be finite");
+        drop_graphs();  // the geometry buffers below are reallocated
         HostGeom hg;
         build_geometry(hg, n, nt, M, theta, phi, center, GMAX);
         fgs = hg.fgs;
@@ -639,9 +674,20 @@ struct Solver {
         RTV_HIP(hipMemsetAsync(d_psi.p, 0, d_psi.n * sizeof(float4), st));
         RTV_HIP(hipMemsetAsync(d_mu.p, 0, d_mu.n * sizeof(float4), st));
         RTV_HIP(hipMemsetAsync(d_dchi.p, 0, d_dchi.n * sizeof(float), st));  // psi = mu = 0
+        static const float rho0 = 0.5f;  // rectv.cu:125
+        static const int zero = 0;
+        RTV_HIP(hipMemcpyAsync(d_rho.p, &rho0, sizeof(float), hipMemcpyHostToDevice, st));
+        RTV_HIP(hipMemcpyAsync(d_iter.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
         ia = 0;
-        rho = 0.5f;
+        iters_done = 0;
         RTV_HIP(hipStreamSynchronize(st));
+    }
+
+    void drop_graphs() {
+        for (auto &g : gexec) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
     }
 
     // ------------------------------------------------------------ operator pipeline
@@ -670,16 +716,7 @@ struct Solver {
         }
         {
             if (!gather_valu) {
-                const int blocks = nfm_entries * ((zc + GM_ZG - 1) / GM_ZG);
-                if (blocks > 0)
-                    launch("gather", [&] {
-                        if (MP <= 8)
-                            hipLaunchKernelGGL(k_gather_mfma<1>, dim3(blocks), dim3(512), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                               d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc);
-                        else
-                            hipLaunchKernelGGL(k_gather_mfma<2>, dim3(blocks), dim3(512), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                               d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc);
-                    });
+                gather(zc, gather_zg);
             } else {
             const int blocks = nf_entries * zc;
             if (blocks > 0)
@@ -699,6 +736,37 @@ struct Solver {
         ray(zc, gdat, nullptr, h1out, mode, partial);
     }
 
+    void gather(int zc, int zg) {
+        const int blocks = nfm_entries * ((zc + zg - 1) / zg);
+        if (blocks > 0)
+            launch("gather", [&] {
+                if (MP <= 8)
+                    hipLaunchKernelGGL(k_gather_mfma<1>, dim3(blocks), dim3(512), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc, zg);
+                else
+                    hipLaunchKernelGGL(k_gather_mfma<2>, dim3(blocks), dim3(512), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc, zg);
+            });
+    }
+
+    template <int ZW>
+    void scatter_t(int zc) {
+        const int blocks = na_tiles * ((zc + ZW - 1) / ZW);
+        launch("scatter", [&] {
+            // MP <= 16 (M <= 32): one or two 16-column blocks
+            if (MP <= 8)
+                hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                   d_asids.p, d_atiles.p, d_arange.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
+            else
+                hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p,
+                                   d_asids.p, d_atiles.p, d_arange.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
+        });
+    }
+    void scatter(int zc, int zw) {
+        if (zw == 2) scatter_t<2>(zc);
+        else scatter_t<4>(zc);
+    }
+
     void ray(int zc, const float *gdat, const float *h1in, float *h1out, int mode, double *partial) {
         const int nrays = zc * nt;
         const int blocks = (nrays + T_ray - 1) / T_ray;
@@ -712,18 +780,7 @@ struct Solver {
 
     // adjoint: d_gq (q samples of zc slices) -> d_v images
     void rays_to_images(int zc) {
-        {
-            const int blocks = na_tiles * ((zc + SM_ZW - 1) / SM_ZW);
-            launch("scatter", [&] {
-                // MP <= 16 (M <= 32): one or two 16-column blocks
-                if (MP <= 8)
-                    hipLaunchKernelGGL(k_scatter<1>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
-                                       d_atiles.p, d_arange.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
-                else
-                    hipLaunchKernelGGL(k_scatter<2>, dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p, d_asids.p,
-                                       d_atiles.p, d_arange.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
-            });
-        }
+        scatter(zc, scatter_zw);
         {
             const int blocks = zc * MP * (N / T_cols);
             const int thr = fft_threads(T_cols, N);
@@ -757,7 +814,7 @@ struct Solver {
         a.gt = (float)(1.0 / c2);
         a.ks = (float)(1.0 / (2.0 * c2 * lam1));
         a.kt = (float)(1.0 / (2.0 * c2));
-        a.rho = rho;
+        a.rho = d_rho.p;
         a.lam0 = lam0;
         return a;
     }
@@ -782,7 +839,10 @@ struct Solver {
             hipLaunchKernelGGL(k_prox<MM>, grid, dim3(256), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
         });
         const int cnt = grid.x * grid.y * grid.z;
-        launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, 3); });
+        launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, PROX_W); });
+        launch("rho", [&] {
+            hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, st, d_red.p, d_rho.p, d_hist.p, d_iter.p, HIST_CAP);
+        });
     }
     template <int MM>
     void launch_divchi(const TvArgs &ta) {
@@ -803,50 +863,127 @@ struct Solver {
     }
 
     // ------------------------------------------------------------ ADMM driver
-    void iterate(int niter, int titer, float lam0, float lam1, double *hist) {
+    // Rotation of the three f buffers over one outer iteration: inner step k reads
+    // `cur` and writes `nxt`; the previous iterate A = d_f[ia] stays untouched
+    // (halo slabs and the prox's f_old).
+    static int next_ia(int ia0, int titer) {
+        const int w1 = (ia0 + 1) % 3, w2 = (ia0 + 2) % 3;
+        int cur = ia0;
+        for (int k = 0; k < titer; k++) cur = (k == 0) ? w1 : (cur == w1 ? w2 : w1);
+        return cur;
+    }
+
+    // One outer ADMM iteration (rectv.cu:140-246) enqueued on the stream: titer
+    // inner steps over all chunks, prox/dual update with the norms, the rho rule
+    // (k_rho_rule) and the dual part of the next iteration's inner steps.
+    void one_iteration(int titer, float lam0, float lam1) {
+        const size_t S = (size_t)M * n * n;
+        const TvArgs ta = tv_args(lam1, lam0);
+        const int w1 = (ia + 1) % 3, w2 = (ia + 2) % 3;
+        float *A = d_f[ia].p;
+        int cur = ia;
+        for (int k = 0; k < titer; k++) {
+            const int nxt = (k == 0) ? w1 : (cur == w1 ? w2 : w1);
+            for (int c0 = 0; c0 < nch; c0 += cb) {
+                const int ncb = std::min(cb, nch - c0);
+                const int z0 = c0 * nzp, zc = ncb * nzp;
+                forward_to_rays(d_f[cur].p + z0 * S, zc, d_g.p + (size_t)z0 * nt * n, nullptr,
+                                RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);
+                rays_to_images(zc);
+                TvArgs tb = ta;
+                tb.chunk0 = c0;
+                update(d_f[cur].p, A, d_f[nxt].p, tb, ncb, z0);
+            }
+            cur = nxt;
+        }
+        prox(d_f[cur].p, A, ta);  // + reduce + rho rule
+        divchi(ta);               // reads the updated rho
+        ia = cur;
+    }
+
+    void graph_iteration(int titer, float lam0, float lam1) {
+        hipGraphExec_t &ge = gexec[ia];
+        if (ge && (gk_titer[ia] != titer || gk_lam0[ia] != lam0 || gk_lam1[ia] != lam1)) {
+            (void)hipGraphExecDestroy(ge);
+            ge = nullptr;
+        }
+        if (!ge) {
+            const int ia0 = ia;
+            hipGraph_t g = nullptr;
+            RTV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                one_iteration(titer, lam0, lam1);
+            } catch (...) {
+                hipGraph_t junk = nullptr;
+                (void)hipStreamEndCapture(st, &junk);
+                if (junk) (void)hipGraphDestroy(junk);
+                ia = ia0;
+                throw;
+            }
+            RTV_HIP(hipStreamEndCapture(st, &g));
+            ia = ia0;
+            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ge = nullptr;
+                RTV_HIP(e);
+            }
+            gk_titer[ia] = titer;
+            gk_lam0[ia] = lam0;
+            gk_lam1[ia] = lam1;
+        }
+        RTV_HIP(hipGraphLaunch(ge, st));
+        ia = next_ia(ia, titer);
+    }
+
+    // Enqueue niter outer iterations; no host synchronisation.
+    void enqueue(int niter, int titer, float lam0, float lam1) {
         RTV_REQUIRE(inputs, "inputs not set (call set_inputs / run first)");
         RTV_REQUIRE(niter >= 0 && titer >= 0, "niter and titer must be >= 0");
         RTV_REQUIRE(std::isfinite(lam0) && std::isfinite(lam1) && lam1 != 0.f, "lambda1 must be finite and nonzero");
-        const size_t S = (size_t)M * n * n;
-        const bool quiet = std::getenv("RECTV_QUIET") && std::atoi(std::getenv("RECTV_QUIET")) != 0;
-        double red[3];
         for (int it = 0; it < niter; it++) {
-            TvArgs ta = tv_args(lam1, lam0);
-            const int w1 = (ia + 1) % 3, w2 = (ia + 2) % 3;
-            float *A = d_f[ia].p;
-            int cur = ia;
-            for (int k = 0; k < titer; k++) {
-                const int nxt = (k == 0) ? w1 : (cur == w1 ? w2 : w1);
-                for (int c0 = 0; c0 < nch; c0 += cb) {
-                    const int ncb = std::min(cb, nch - c0);
-                    const int z0 = c0 * nzp, zc = ncb * nzp;
-                    forward_to_rays(d_f[cur].p + z0 * S, zc, d_g.p + (size_t)z0 * nt * n, nullptr,
-                                    RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);
-                    rays_to_images(zc);
-                    TvArgs tb = ta;
-                    tb.chunk0 = c0;
-                    update(d_f[cur].p, A, d_f[nxt].p, tb, ncb, z0);
-                }
-                cur = nxt;
-            }
-            prox(d_f[cur].p, A, ta);
-            RTV_HIP(hipMemcpyAsync(red, d_red.p, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
-            RTV_HIP(hipStreamSynchronize(st));
+            if (use_graph && !prof) graph_iteration(titer, lam0, lam1);
+            else one_iteration(titer, lam0, lam1);
+            iters_done++;
+        }
+    }
+
+    // Rows [first, first + count) of the per-iteration norms (waits for the stream).
+    void history(long long first, int count, double *out) {
+        RTV_REQUIRE(out || count == 0, "null history buffer");
+        RTV_REQUIRE(first >= 0 && count >= 0 && first + count <= iters_done,
+                    "history rows requested beyond the iterations run since reset");
+        RTV_REQUIRE(iters_done - first <= HIST_CAP, "history rows no longer held (ring of 1024 iterations)");
+        if (count == 0) return;
+        std::vector<double> ring((size_t)HIST_CAP * 5);
+        RTV_HIP(hipMemcpyAsync(ring.data(), d_hist.p, ring.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        RTV_HIP(hipStreamSynchronize(st));
+        for (int q = 0; q < count; q++) {
+            const size_t row = (size_t)((first + q) % HIST_CAP);
+            for (int c = 0; c < 5; c++) out[5 * (size_t)q + c] = ring[5 * row + c];
+        }
+    }
+
+    void iterate(int niter, int titer, float lam0, float lam1, double *hist) {
+        const bool quiet = std::getenv("RECTV_QUIET") && std::atoi(std::getenv("RECTV_QUIET")) != 0;
+        RTV_REQUIRE(niter >= 0, "niter must be >= 0");
+        int done = 0;
+        std::vector<double> h;
+        while (done < niter) {
+            // the reference prints one line per outer iteration (rectv.cu:238)
+            const int blk = quiet ? std::min(HIST_CAP, niter - done) : 1;
+            const long long first = iters_done;
+            enqueue(blk, titer, lam0, lam1);
+            h.resize((size_t)5 * blk);
+            history(first, blk, h.data());
             collect();
-            const double r = red[0], s = (double)rho * rho * red[1], rdiff = red[2];
-            if (hist) {
-                hist[4 * it + 0] = r;
-                hist[4 * it + 1] = s;
-                hist[4 * it + 2] = rho;
-                hist[4 * it + 3] = rdiff;
+            for (int q = 0; q < blk; q++) {
+                const int it = done + q;
+                if (hist)
+                    for (int c = 0; c < 4; c++) hist[4 * (size_t)it + c] = h[5 * (size_t)q + c];
+                if (!quiet) fprintf(stderr, "iter (%d/%d), rdiff ||f(k+1)-f(k)||:%f \n", it, niter, h[5 * (size_t)q + 3]);
             }
-            // rho rule, rectv.cu:224-232
-            if (r > 10 * s) rho *= 2;
-            else if (s > 10 * r) rho *= 0.5f;
-            if (!quiet) fprintf(stderr, "iter (%d/%d), rdiff ||f(k+1)-f(k)||:%f \n", it, niter, rdiff);
-            ia = cur;
-            // dual part of the next outer iteration's inner steps, with the new rho
-            divchi(tv_args(lam1, lam0));
+            done += blk;
         }
     }
 
@@ -950,6 +1087,89 @@ struct Solver {
         th.release(tmp);
     }
 
+    // ------------------------------------------------------------ autotuner
+    // Chooses the slices per workgroup of the gather (zg) and of the scatter (zw)
+    // for this problem shape by timing each candidate on the solver's own
+    // geometry (HIP events, best of 3 after a warm-up launch).  Every candidate
+    // does the same arithmetic in the same order, so results do not depend on
+    // the choice.  Choices are cached per shape for the life of the process.
+    struct TuneKey {
+        int n, nt, M, zb;
+        bool operator<(const TuneKey &o) const {
+            return std::tie(n, nt, M, zb) < std::tie(o.n, o.nt, o.M, o.zb);
+        }
+    };
+    static std::map<TuneKey, std::pair<int, int>> &tune_cache() {
+        static std::map<TuneKey, std::pair<int, int>> c;
+        return c;
+    }
+    static std::mutex &tune_mutex() {
+        static std::mutex m;
+        return m;
+    }
+    template <typename F>
+    float time_best(F &&f) {
+        hipEvent_t a = ev(), b = ev();
+        f();  // warm-up
+        float best = 1e30f;
+        for (int r = 0; r < 3; r++) {
+            RTV_HIP(hipEventRecord(a, st));
+            f();
+            RTV_HIP(hipEventRecord(b, st));
+            RTV_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            RTV_HIP(hipEventElapsedTime(&ms, a, b));
+            best = std::min(best, ms);
+        }
+        evpool.push_back(a);
+        evpool.push_back(b);
+        return best;
+    }
+    void autotune(int *zg_out, int *zw_out) {
+        RTV_REQUIRE(inputs, "inputs not set (call set_inputs first)");
+        const TuneKey key{n, nt, M, zb};
+        std::pair<int, int> choice{-1, -1};
+        {
+            std::lock_guard<std::mutex> lk(tune_mutex());
+            auto it = tune_cache().find(key);
+            if (it != tune_cache().end()) choice = it->second;
+        }
+        if (choice.first < 0) {
+            const bool was = prof;
+            prof = false;
+            const int zc = std::min(zb, nz);
+            // real spectra and samples of the current iterate in G and gq
+            forward_to_rays(d_f[ia].p, zc, d_g.p, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);
+            float bestg = 1e30f, bests = 1e30f;
+            int zg = GM_ZG, zw = SM_ZW;
+            for (int c : {1, 2, 4, 8, 16}) {
+                if (c > 1 && c / 2 >= zc) break;
+                const float t = time_best([&] { gather(zc, c); });
+                if (t < bestg) {
+                    bestg = t;
+                    zg = c;
+                }
+            }
+            ray(zc, d_g.p, nullptr, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);  // q back in gq
+            for (int c : {2, 4}) {
+                const float t = time_best([&] { scatter(zc, c); });
+                if (t < bests) {
+                    bests = t;
+                    zw = c;
+                }
+            }
+            prof = was;
+            choice = {zg, zw};
+            std::lock_guard<std::mutex> lk(tune_mutex());
+            tune_cache()[key] = choice;
+        }
+        gather_zg = choice.first;
+        scatter_zw = choice.second;
+        drop_graphs();
+        if (zg_out) *zg_out = gather_zg;
+        if (zw_out) *zw_out = scatter_zw;
+    }
+
     // rectv::check_adjoints, rectv.cu:255-289
     void check_adjoints(double *res, const float *g, const float *theta, const float *phi, float lam1, float center) {
         RTV_REQUIRE(nz == nzp, "check_adjoints requires nz == nzp (one chunk), as in the reference");
@@ -986,8 +1206,12 @@ struct Solver {
         pending.clear();
         for (auto e : evpool) (void)hipEventDestroy(e);
         evpool.clear();
+        drop_graphs();
         size_t &t = bytes;
         d_tw.release(t);
+        d_rho.release(t);
+        d_hist.release(t);
+        d_iter.release(t);
         for (auto &b : d_f) b.release(t);
         d_psi.release(t);
         d_mu.release(t);
@@ -1154,6 +1378,72 @@ int rtv_get_profile(rtv_solver *h, char *buf, int len) {
 long long rtv_device_bytes(rtv_solver *h) {
     if (!h || !h->s) return -1;
     return (long long)h->s->bytes;
+}
+
+int rtv_iterate_async(rtv_solver *h, int niter, int titer, float lambda0, float lambda1) {
+    return guarded([&] { live(h)->enqueue(niter, titer, lambda0, lambda1); });
+}
+
+int rtv_query(rtv_solver *h, int *idle) {
+    return guarded([&] {
+        RTV_REQUIRE(idle, "null output pointer");
+        const hipError_t e = hipStreamQuery(live(h)->st);
+        if (e == hipSuccess) *idle = 1;
+        else if (e == hipErrorNotReady) *idle = 0;
+        else RTV_HIP(e);
+    });
+}
+
+int rtv_iterations(rtv_solver *h, long long *count) {
+    return guarded([&] {
+        RTV_REQUIRE(count, "null output pointer");
+        *count = live(h)->iters_done;
+    });
+}
+
+int rtv_history(rtv_solver *h, long long first, int count, double *rows) {
+    return guarded([&] { live(h)->history(first, count, rows); });
+}
+
+int rtv_set_graph(rtv_solver *h, int on) {
+    return guarded([&] {
+        rtv::Solver *s = live(h);
+        s->use_graph = on != 0;
+        if (!on) s->drop_graphs();
+    });
+}
+
+int rtv_autotune(rtv_solver *h, int *gather_slices, int *scatter_slices) {
+    return guarded([&] { live(h)->autotune(gather_slices, scatter_slices); });
+}
+
+int rtv_batch_iterate(rtv_solver **hs, int count, const int *niter, const int *titer, const float *lambda0,
+                      const float *lambda1) {
+    return guarded([&] {
+        RTV_REQUIRE(count >= 0 && (count == 0 || (hs && niter && titer && lambda0 && lambda1)), "null batch arrays");
+        std::vector<rtv::Solver *> ss(count);
+        int most = 0;
+        for (int j = 0; j < count; j++) {
+            ss[j] = live(hs[j]);
+            for (int i = 0; i < j; i++) RTV_REQUIRE(ss[i] != ss[j], "a solver appears twice in the batch");
+            RTV_REQUIRE(niter[j] >= 0, "niter must be >= 0");
+            most = std::max(most, niter[j]);
+        }
+        // round robin: iteration k of every job is enqueued (each on its own
+        // stream) before iteration k + 1 of any, so the jobs' kernels overlap
+        for (int k = 0; k < most; k++)
+            for (int j = 0; j < count; j++)
+                if (k < niter[j]) ss[j]->enqueue(1, titer[j], lambda0[j], lambda1[j]);
+        for (int j = 0; j < count; j++) RTV_HIP(hipStreamSynchronize(ss[j]->st));
+    });
+}
+
+int rtv_footprint(int n, int ntheta, int m, int nz, int nzp, long long *bytes) {
+    return guarded([&] {
+        RTV_REQUIRE(bytes, "null output pointer");
+        RTV_REQUIRE(n > 0 && ntheta > 0 && m > 0 && nz > 0 && nzp > 0 && nz % nzp == 0, "invalid sizes");
+        *bytes = (long long)rtv::Solver::plan_bytes(n, ntheta, m, nz, nzp);
+    });
 }
 
 }  // extern "C"

@@ -97,6 +97,48 @@ class Solver(rectv):
                                           float(lambda1), getp(hist) if history else None))
         return hist[:niter] if history else None
 
+    def iterate_async(self, niter, titer, lambda0, lambda1):
+        """Enqueue niter outer iterations on the solver's stream and return at once."""
+        _lib.check(_lib.lib().rtv_iterate_async(self._handle(), int(niter), int(titer), float(lambda0),
+                                                float(lambda1)))
+
+    def idle(self):
+        """True when every enqueued iteration has finished."""
+        v = ctypes.c_int()
+        _lib.check(_lib.lib().rtv_query(self._handle(), ctypes.byref(v)))
+        return bool(v.value)
+
+    def iterations(self):
+        """Outer iterations enqueued since the last reset."""
+        v = ctypes.c_longlong()
+        _lib.check(_lib.lib().rtv_iterations(self._handle(), ctypes.byref(v)))
+        return int(v.value)
+
+    def history(self, first, count):
+        """Rows [first, first+count) since reset as [count, 5]: r, s, rho used,
+        ||f(k+1)-f(k)||^2, ||f(k+1)||^2 (waits for the solver's stream)."""
+        out = np.zeros((max(count, 1), 5))
+        _lib.check(_lib.lib().rtv_history(self._handle(), int(first), int(count), getp(out)))
+        return out[:count]
+
+    def set_graph(self, on=True):
+        """Capture each outer iteration once as a HIP graph and replay it."""
+        _lib.check(_lib.lib().rtv_set_graph(self._handle(), 1 if on else 0))
+
+    def autotune(self):
+        """Pick the gather/scatter launch shapes for this geometry by timing them;
+        returns (gather slices per workgroup, scatter slices per workgroup)."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().rtv_autotune(self._handle(), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    @staticmethod
+    def footprint(n, ntheta, m, nz, nzp):
+        """Upper bound of the device bytes a solver of these sizes holds."""
+        v = ctypes.c_longlong()
+        _lib.check(_lib.lib().rtv_footprint(int(n), int(ntheta), int(m), int(nz), int(nzp), ctypes.byref(v)))
+        return int(v.value)
+
     def result(self):
         out = np.zeros([self.nz, self.m, self.n, self.n], dtype='float32')
         _lib.check(_lib.lib().rtv_get_result(self._handle(), getp(out)))

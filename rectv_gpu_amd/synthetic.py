@@ -86,3 +86,41 @@ def workload(cfg, seed=0):
     data = project(sph, n, nz, theta, ntr, m, noise=nz_noise, seed=seed)
     return dict(n=n, nz=nz, nzp=nzp, ntheta=nrot * ntr, m=m, theta=theta, phi=phi, data=data,
                 niter=niter, lambda0=1e-3, lambda1=4.0, center=n / 2, nrot=nrot, ntheta_rot=ntr)
+
+
+def batch_specs(count=64, seed0=1, rng_seed=4):
+    """Sizes of the batched-instances mix: N from {128, 256, 384, 512}, Nz from
+    U{4..32}, projections per rotation from {64, 100, 128}, rotations from
+    {2, 4, 8}, M from {4, 8, 16}; job i uses phantom seed seed0 + i; Nzp = Nz."""
+    rng = np.random.default_rng(rng_seed)
+    out = []
+    for i in range(count):
+        nz = int(rng.integers(4, 33))
+        out.append(dict(n=int(rng.choice([128, 256, 384, 512])), nz=nz, nzp=nz,
+                        ntr=int(rng.choice([64, 100, 128])), nrot=int(rng.choice([2, 4, 8])),
+                        m=int(rng.choice([4, 8, 16])), seed=seed0 + i))
+    return out
+
+
+def serve_specs(count=64, seed0=1001, rng_seed=5, big_frac=0.05):
+    """Sizes of the serving mix: the batched mix plus a fraction big_frac of
+    N=1024, Nz=32, M=16 jobs (4 rotations x 200 projections)."""
+    rng = np.random.default_rng(rng_seed)
+    base = batch_specs(count, seed0, rng_seed + 100)
+    for i, sp in enumerate(base):
+        if rng.random() < big_frac:
+            base[i] = dict(n=1024, nz=32, nzp=32, ntr=200, nrot=4, m=16, seed=sp["seed"])
+    return base
+
+
+def make_job(spec, niter, titer, spheres_count=30, noise=0.02, tol=None):
+    """A batch.Job of the given sizes: moving-sphere phantom, Fourier basis,
+    lambda0=1e-3, lambda1=4, centre n/2."""
+    from .basis import angles, takephi
+    from .batch import Job
+    n, nz, ntr, nrot, m = spec["n"], spec["nz"], spec["ntr"], spec["nrot"], spec["m"]
+    theta = angles(ntr, nrot)
+    sph = spheres(n, nz, spheres_count, spec["seed"], nrot=nrot)
+    data = project(sph, n, nz, theta, ntr, m, noise=noise, seed=spec["seed"])
+    return Job(data, theta, takephi(nrot * ntr, m), lambda0=1e-3, lambda1=4.0, niter=niter, titer=titer,
+               nzp=spec.get("nzp", nz), tol=tol, name="n%d_nz%d_nt%d_m%d_s%d" % (n, nz, nrot * ntr, m, spec["seed"]))
