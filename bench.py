@@ -122,8 +122,8 @@ def batched_bench(count, niter, titer, workers=8):
     host set-up (geometry tables, uploads) included in both."""
     from rectv_gpu_amd import synthetic, batch
     specs = synthetic.batch_specs(count)
-    mk = lambda: [synthetic.make_job(sp, niter, titer) for sp in specs]
-    jobs_a = mk()
+    jobs0 = [synthetic.make_job(sp, niter, titer) for sp in specs]
+    jobs_a = [j.fresh() for j in jobs0]
     vox = sum(j.voxels * j.niter for j in jobs_a)
     print("bench: batched mix of %d jobs generated (%.2f G voxel-iterations)" % (count, vox / 1e9),
           file=sys.stderr, flush=True)
@@ -133,7 +133,7 @@ def batched_bench(count, niter, titer, workers=8):
     print("bench: sequential plain solves %.2f s" % t_seq, file=sys.stderr, flush=True)
     ref = [j.result for j in jobs_a]
     del jobs_a
-    jobs_b = mk()
+    jobs_b = [j.fresh() for j in jobs0]
     t0 = time.perf_counter()
     batch.solve_batch(jobs_b, workers=workers)
     t_bat = time.perf_counter() - t0
@@ -188,7 +188,7 @@ def served_bench(count, niter, titer, rate, seed=7, max_active=16, sample_frac=0
     sample = sorted(range(count), key=lambda i: jobs[i].voxels)[:k]
     diff = 0.0
     for i in sample:
-        r = batch.solve_sequential([synthetic.make_job(specs[i], niter, titer)])[0].result
+        r = batch.solve_sequential([jobs[i].fresh()])[0].result
         diff = max(diff, float(np.linalg.norm(jobs[i].result.astype(np.float64) - r) / max(np.linalg.norm(r), 1e-30)))
     return {"workload": "%d jobs (batched mix + 5%% N=1024 Nz=32 M=16), Poisson arrivals %.0f jobs/s, niter=%d, "
                         "titer=%d, max %d active" % (count, rate, niter, titer, max_active),

@@ -7,10 +7,12 @@ centre defaults to n/2) and an optional early stop (``tol``) on the relative
 change ||f(k+1) - f(k)|| / ||f(k+1)|| between outer iterations.
 
 ``solve_batch`` runs a list of uneven jobs together: solvers are prepared on a
-pool of host threads (the host builds each job's geometry tables while the
-others upload), every job's outer iteration is captured once as a HIP graph,
-and the iterations of all jobs are enqueued round robin on their own streams
-(``rtv_batch_iterate``) so that their kernels overlap on the device.
+pool of host threads (the host builds a job's geometry tables while earlier
+jobs already iterate), every job's outer iteration is captured once as a HIP
+graph, and the iterations of all jobs are enqueued a few at a time on their own
+streams so that their kernels overlap on the device.  ``batch_iterate`` (the C
+entry point ``rtv_batch_iterate``) does the same round robin for solvers a
+caller has already prepared.
 
 ``Server`` is the serving form: jobs arrive at any time (``submit``), are
 admitted while their device footprint fits the memory budget and the number
@@ -63,6 +65,11 @@ class Job:
         self.t_submit = self.t_admit = self.t_start = self.t_done = None
         self.done = threading.Event()
 
+    def fresh(self):
+        """A new, unsolved job with the same inputs and parameters."""
+        return Job(self.data, self.theta, self.phi, self.rot_center, self.lambda0, self.lambda1, self.niter,
+                   self.titer, self.nzp, self.tol, self.name)
+
     @property
     def voxels(self):
         return self.n * self.n * self.nz * self.m
@@ -114,47 +121,34 @@ def finish(job, s):
     s.free()
 
 
-def solve_batch(jobs, workers=8, graph=True, autotune=True, budget=None):
-    """Solve uneven jobs together (see the module docstring).  Jobs whose
-    footprints together exceed ``budget`` bytes (default: 90% of free device
-    memory) run in successive groups.  Returns the jobs with results set."""
-    L = _lib.lib()
-    if budget is None:
-        budget = 0.9 * device_free_bytes()
-    groups, cur, used = [], [], 0
-    for j in jobs:
-        fp = j.footprint()
-        if cur and used + fp > budget:
-            groups.append(cur)
-            cur, used = [], 0
-        cur.append(j)
-        used += fp
-    if cur:
-        groups.append(cur)
-    with cf.ThreadPoolExecutor(max_workers=workers) as pool:
-        for grp in groups:
-            for j in grp:
-                j.t_submit = j.t_admit = time.perf_counter()
-            solvers = list(pool.map(lambda j: prepare(j, graph=graph, autotune=autotune), grp))
-            t0 = time.perf_counter()
-            for j in grp:
-                j.t_start = t0
-            try:
-                k = len(grp)
-                hs = (ctypes.c_void_p * k)(*[s._handle().value for s in solvers])
-                ni = np.array([j.niter for j in grp], np.int32)
-                ti = np.array([j.titer for j in grp], np.int32)
-                l0 = np.array([j.lambda0 * j.lambda1 for j in grp], np.float32)
-                l1 = np.array([j.lambda1 for j in grp], np.float32)
-                _lib.check(L.rtv_batch_iterate(hs, k, ni.ctypes.data, ti.ctypes.data, l0.ctypes.data, l1.ctypes.data))
-                for j, s in zip(grp, solvers):
-                    finish(j, s)
-                    j.t_done = time.perf_counter()
-                    j.done.set()
-            finally:
-                for s in solvers:
-                    s.free()
+def solve_batch(jobs, workers=8, graph=True, autotune=True, budget=None, ahead=4):
+    """Solve uneven jobs together (see the module docstring): all jobs are
+    submitted at once to a Server with a slot for every job, so host
+    preparation (geometry tables, uploads) of later jobs overlaps the device
+    work of earlier ones and the jobs' iterations are interleaved on their own
+    streams.  Jobs whose footprints do not fit ``budget`` (default: 90% of
+    free device memory) wait for memory like served jobs.  Returns the jobs."""
+    srv = Server(budget=budget, max_active=max(1, len(jobs)), ahead=ahead, workers=workers, graph=graph,
+                 autotune=autotune)
+    try:
+        for j in jobs:
+            srv.submit(j)
+        srv.run()
+    finally:
+        srv.close()
     return jobs
+
+
+def batch_iterate(solvers, niter, titer, lambda0, lambda1):
+    """rtv_batch_iterate over prepared solvers (inputs set, state reset):
+    niter[j] outer iterations of each, interleaved round robin on their own
+    streams; lambda0 is used as given (already multiplied by lambda1)."""
+    k = len(solvers)
+    hs = (ctypes.c_void_p * k)(*[s._handle().value for s in solvers])
+    arr = lambda v, dt: np.ascontiguousarray(np.broadcast_to(np.asarray(v, dt), (k,)))
+    ni, ti = arr(niter, np.int32), arr(titer, np.int32)
+    l0, l1 = arr(lambda0, np.float32), arr(lambda1, np.float32)
+    _lib.check(_lib.lib().rtv_batch_iterate(hs, k, ni.ctypes.data, ti.ctypes.data, l0.ctypes.data, l1.ctypes.data))
 
 
 def solve_sequential(jobs):

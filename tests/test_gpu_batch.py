@@ -108,6 +108,20 @@ def test_batch_matches_alone(gpu):
         say("  ", j.name, e, "iterations", j.iterations)
         assert j.error is None and j.iterations == j.niter
         assert e < SAME_TOL
+    # the C round robin over prepared solvers (rtv_batch_iterate)
+    from rectv_gpu_amd.batch import prepare, batch_iterate
+    sol = [prepare(j, autotune=False) for j in jobs]
+    try:
+        batch_iterate(sol, [j.niter for j in jobs], [j.titer for j in jobs],
+                      [j.lambda0 * j.lambda1 for j in jobs], [j.lambda1 for j in jobs])
+        for s, j, r in zip(sol, jobs, ref):
+            assert s.iterations() == j.niter
+            e = rel(s.result(), r)
+            say("  round robin", j.name, e)
+            assert e < SAME_TOL
+    finally:
+        for s in sol:
+            s.free()
 
 
 def test_server_mix_matches_alone_and_oracle(gpu):
