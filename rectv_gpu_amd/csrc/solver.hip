@@ -66,7 +66,11 @@ struct HostGeom {
     std::vector<int> fitems;  // forward items per origin tile for the matrix-core gather (-1 = padding)
     std::vector<int> asids;
     std::vector<TileEntry> ftiles, fmtiles, atiles;  // atiles[].pad: first chunk of the tile
-    std::vector<int2> arange;  // per (adjoint chunk, tile row): [lo, hi) of the items covering the row
+    // adjoint chunks (SM_CH items of a tile): crange = (byte offset, byte length) of the
+    // chunk's block lists in blist; brange per (chunk, 4x4 block) = (offset within the
+    // chunk's lists, length): slots of the items whose window meets the block
+    std::vector<int2> crange, brange;
+    std::vector<unsigned char> blist;
     std::vector<float2> pre_f, post_a, cf;
     std::vector<float> post_f, pre_a, deap;
     float beta = 0.f;
@@ -278,29 +282,38 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
         hg.atiles.resize(ntile);
         for (int t = 0; t < ntile; t++) hg.atiles[t] = TileEntry{t, off[t], cnt[t], 0};
-        // per chunk of SM_CH items and tile row r: the items whose window rows
-        // ly0 .. ly0+8 contain r form one run (items are sorted by ly0)
-        hg.arange.clear();
+        // per chunk of SM_CH items and 4 x 4 cell block: the slots of the items whose
+        // 9 x 9 window meets the block, ascending, padded to a multiple of 4 with the
+        // zero slot SM_CH
+        hg.crange.clear();
+        hg.brange.clear();
+        hg.blist.clear();
         for (int t = 0; t < ntile; t++) {
             TileEntry &te = hg.atiles[t];
-            te.pad = (int)(hg.arange.size() / TS);
-            const int ty0 = (t / nT) * TS;
+            te.pad = (int)hg.crange.size();
+            const int ty0 = (t / nT) * TS, tx0 = (t % nT) * TS;
             for (int c0 = 0; c0 < te.count; c0 += SM_CH) {
                 const int cn = std::min(SM_CH, te.count - c0);
-                std::vector<int> ly(cn);
+                std::vector<int> ly(cn), lx(cn);
                 for (int i = 0; i < cn; i++) {
-                    const int dy = (hg.geo[hg.asids[te.start + c0 + i]].row0 - ty0 + N) % N;
+                    const SampleGeom &g = hg.geo[hg.asids[te.start + c0 + i]];
+                    const int dy = (g.row0 - ty0 + N) % N, dx = (g.col0 - tx0 + N) % N;
                     ly[i] = dy < TS ? dy : dy - N;
+                    lx[i] = dx < TS ? dx : dx - N;
                 }
-                for (int r = 0; r < TS; r++) {
-                    int lo = cn, hi = 0;
+                const int base = (int)hg.blist.size();
+                for (int b = 0; b < 16; b++) {
+                    const int r0 = 4 * (b / 4), q0 = 4 * (b % 4);
+                    const int off = (int)hg.blist.size() - base;
                     for (int i = 0; i < cn; i++)
-                        if (ly[i] <= r && r <= ly[i] + KW - 1) {
-                            lo = std::min(lo, i);
-                            hi = std::max(hi, i + 1);
-                        }
-                    hg.arange.push_back(make_int2(lo, hi));
+                        if (ly[i] <= r0 + 3 && ly[i] + KW - 1 >= r0 && lx[i] <= q0 + 3 && lx[i] + KW - 1 >= q0)
+                            hg.blist.push_back((unsigned char)i);
+                    while (((int)hg.blist.size() - base - off) % 4) hg.blist.push_back((unsigned char)SM_CH);
+                    hg.brange.push_back(make_int2(off, (int)hg.blist.size() - base - off));
                 }
+                const int len = (int)hg.blist.size() - base;
+                if (len > SM_LB) throw Error(-1, "internal: scatter block lists exceed their staging area");
+                hg.crange.push_back(make_int2(base, len));
             }
         }
         if (std::getenv("RECTV_DENSE_FIRST"))
@@ -440,7 +453,8 @@ struct Solver {
     DBuf<int> d_fitems;
     DBuf<int> d_asids;
     DBuf<TileEntry> d_ftiles, d_fmtiles, d_atiles;
-    DBuf<int2> d_arange;
+    DBuf<int2> d_crange, d_brange;
+    DBuf<unsigned char> d_blist;
     DBuf<float2> d_cf, d_pre_f, d_post_a;
     DBuf<float> d_post_f, d_pre_a, d_deap;
     int nf_entries = 0, nfm_entries = 0, na_tiles = 0, fgs = 4;
@@ -481,9 +495,11 @@ struct Solver {
         const size_t state = 3 * V * 4 + 2 * V4 * 16 + V * 4 + (size_t)nz * nt * n * 4;
         const size_t per_slice = MP * n * N * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 16 + (size_t)nt * n * 4;
         const size_t ni = 2 * (size_t)nt * n, ntile = (N / TS) * (N / TS);
-        // geo 16 B, VALU groups <= 16 B, gather slots <= 16 ints, adjoint lists <= 4 ints per item,
-        // row runs 16 int2 per tile and chunk; tables per ray / pixel
-        const size_t geom = ni * (16 + 16 + 64 + 16) + (ntile + 4 * ni / SM_CH + 1) * TS * 8 + 3 * ntile * 16 +
+        // geo 16 B, VALU groups <= 16 B, gather slots <= 16 ints, adjoint lists <= 4 ints and
+        // block lists <= 9 bytes per item; per adjoint chunk its range, 16 block ranges and
+        // up to 3 padding bytes per block; tables per ray / pixel
+        const size_t nchk = ntile + 4 * ni / SM_CH + 1;
+        const size_t geom = ni * (16 + 16 + 64 + 16 + 9) + nchk * (8 + 16 * 8 + 16 * 3) + 3 * ntile * 16 +
                             (size_t)nt * 2 * MP * 8 + 8 * (size_t)n * 8 + 2 * (N + n) * 8;
         return state + per_slice * nz + geom + (1 << 20);
     }
@@ -654,7 +670,9 @@ struct Solver {
         gather_valu = std::getenv("RECTV_GATHER") && std::string(std::getenv("RECTV_GATHER")) == "valu";
         upload(d_asids, hg.asids, bytes, st);
         upload(d_atiles, hg.atiles, bytes, st);
-        upload(d_arange, hg.arange, bytes, st);
+        upload(d_crange, hg.crange, bytes, st);
+        upload(d_brange, hg.brange, bytes, st);
+        upload(d_blist, hg.blist, bytes, st);
         upload(d_cf, hg.cf, bytes, st);
         upload(d_pre_f, hg.pre_f, bytes, st);
         upload(d_post_f, hg.post_f, bytes, st);
@@ -756,10 +774,12 @@ struct Solver {
             // MP <= 16 (M <= 32): one or two 16-column blocks
             if (MP <= 8)
                 hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                   d_asids.p, d_atiles.p, d_arange.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
+                                   d_asids.p, d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, n, nt, MP, beta,
+                                   na_tiles, zc);
             else
                 hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                   d_asids.p, d_atiles.p, d_arange.p, d_cf.p, n, nt, MP, beta, na_tiles, zc);
+                                   d_asids.p, d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, n, nt, MP, beta,
+                                   na_tiles, zc);
         });
     }
     void scatter(int zc, int zw) {
@@ -1231,7 +1251,9 @@ struct Solver {
         d_fitems.release(t);
         d_fmtiles.release(t);
         d_atiles.release(t);
-        d_arange.release(t);
+        d_crange.release(t);
+        d_brange.release(t);
+        d_blist.release(t);
         d_cf.release(t);
         d_pre_f.release(t);
         d_post_a.release(t);
