@@ -1014,6 +1014,22 @@ struct TvArgs {
     float lam0;
 };
 
+// TV kernels: workgroups of 64 x TV_ROWS threads over (x, y) tiles of one chunk.
+// The 1D grid (x tiles fastest, then y tiles, then chunks) is remapped so that
+// consecutive tiles share an XCD and its L2, which then serves the rows and
+// columns that neighbouring tiles read across their edges.
+constexpr int TV_ROWS = 16;
+constexpr int TV_THREADS = 64 * TV_ROWS;
+struct TvTile {
+    int x, y, c, b;  // column, row, chunk, logical workgroup index
+};
+__device__ __forceinline__ TvTile tv_tile(int nx, int ny, int chunk0) {
+    const int gx = (nx + 63) / 64, gy = (ny + TV_ROWS - 1) / TV_ROWS;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int bx = L % gx, by = (L / gx) % gy, bz = L / (gx * gy);
+    return TvTile{bx * 64 + (int)(threadIdx.x & 63), by * TV_ROWS + (int)(threadIdx.x >> 6), chunk0 + bz, L};
+}
+
 __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i, int y, int x) {
     return ((((size_t)c * (a.nzp + 1) + zl) * (a.M + 1) + i) * (a.n + 1) + y) * (a.n + 1) + x;
 }
@@ -1028,11 +1044,10 @@ __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i
 // the rho rule.  Thread per (chunk, y, x) column; the t and z neighbours are the
 // previous frame's / slice's own values and are carried.
 template <int MM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(TV_THREADS)
 k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *__restrict__ dchi, TvArgs a) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int c = a.chunk0 + blockIdx.z;
+    const TvTile tt = tv_tile(a.n, a.n, a.chunk0);
+    const int x = tt.x, y = tt.y, c = tt.c;
     const int n = a.n, M = a.M, nzp = a.nzp;
     if (x >= n || y >= n) return;
     const size_t plane = (size_t)n * n, S = (size_t)M * plane;
@@ -1074,12 +1089,11 @@ k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *_
 // gradient values needed by the backward differences in z and t are the
 // previous slice's / frame's own values, so they are carried.
 template <int MM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(TV_THREADS)
 k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__restrict__ out,
          const float *__restrict__ dchi, const float2 *__restrict__ v, TvArgs a, int zv0) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int c = a.chunk0 + blockIdx.z;
+    const TvTile tt = tv_tile(a.n, a.n, a.chunk0);
+    const int x = tt.x, y = tt.y, c = tt.c;
     const int n = a.n, M = a.M, nzp = a.nzp;
     if (x >= n || y >= n) return;
     const bool flgl = c != 0, flgr = c != a.nchunk - 1;
@@ -1157,13 +1171,12 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 // dual column (chunk, y', x') over all (n+1)^2 positions, ghost entries included.
 constexpr int PROX_W = 4;  // partial sums per workgroup
 template <int MM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(TV_THREADS)
 k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__restrict__ psi,
        float4 *__restrict__ mu, TvArgs a, double *__restrict__ partial) {
-    __shared__ double red[PROX_W][4];
-    const int xq = blockIdx.x * 64 + (threadIdx.x & 63);  // x' in [0, n]
-    const int yq = blockIdx.y * 4 + (threadIdx.x >> 6);   // y' in [0, n]
-    const int c = a.chunk0 + blockIdx.z;
+    __shared__ double red[PROX_W][TV_THREADS / 64];
+    const TvTile tt = tv_tile(a.n + 1, a.n + 1, a.chunk0);
+    const int xq = tt.x, yq = tt.y, c = tt.c;  // dual position (x', y') in [0, n]^2
     const int n = a.n, M = a.M, nzp = a.nzp;
     const float r = *a.rho;
     double sr = 0.0, ss = 0.0, sc = 0.0, sf = 0.0;
@@ -1280,8 +1293,11 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        for (int q = 0; q < PROX_W; q++) partial[PROX_W * b + q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
+        for (int q = 0; q < PROX_W; q++) {
+            double t = 0.0;
+            for (int k = 0; k < TV_THREADS / 64; k++) t += red[q][k];
+            partial[PROX_W * (size_t)tt.b + q] = t;
+        }
     }
 }
 

@@ -543,7 +543,7 @@ struct Solver {
         d_v.alloc((size_t)zb * MP * n * n, bytes);
         d_gq.alloc((size_t)zb * nt * n * 2, bytes);
         d_h1.alloc((size_t)zb * nt * n, bytes);
-        const size_t pb = (size_t)((n + 1 + 63) / 64) * ((n + 1 + 3) / 4) * nch;
+        const size_t pb = (size_t)tv_blocks(n + 1, nch);
         const size_t rb = (size_t)((zb * nt + T_ray - 1) / T_ray);
         d_part.alloc(PROX_W * std::max(pb, rb) + 16, bytes);
         d_red.alloc(8, bytes);
@@ -822,6 +822,9 @@ struct Solver {
         }
     }
 
+    // workgroups of the TV kernels over an m x m (x, y) plane per chunk, for nc chunks
+    static int tv_blocks(int m, int nc) { return ((m + 63) / 64) * ((m + TV_ROWS - 1) / TV_ROWS) * nc; }
+
     TvArgs tv_args(float lam1, float lam0) const {
         TvArgs a;
         a.n = n;
@@ -841,9 +844,9 @@ struct Solver {
 
     template <int MM>
     void launch_update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
-        dim3 grid((n + 63) / 64, (n + 3) / 4, ncb);
+        const dim3 grid(tv_blocks(n, ncb));
         launch("update", [&] {
-            hipLaunchKernelGGL(k_update<MM>, grid, dim3(256), 0, st, cur, A, nxt, d_dchi.p, d_v.p, ta, zv0);
+            hipLaunchKernelGGL(k_update<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, nxt, d_dchi.p, d_v.p, ta, zv0);
         });
     }
     void update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
@@ -854,11 +857,11 @@ struct Solver {
     }
     template <int MM>
     void launch_prox(const float *cur, const float *A, const TvArgs &ta) {
-        dim3 grid((n + 1 + 63) / 64, (n + 1 + 3) / 4, nch);
+        const dim3 grid(tv_blocks(n + 1, nch));
         launch("prox", [&] {
-            hipLaunchKernelGGL(k_prox<MM>, grid, dim3(256), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
+            hipLaunchKernelGGL(k_prox<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
         });
-        const int cnt = grid.x * grid.y * grid.z;
+        const int cnt = grid.x;
         launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, PROX_W); });
         launch("rho", [&] {
             hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, st, d_red.p, d_rho.p, d_hist.p, d_iter.p, HIST_CAP);
@@ -866,8 +869,10 @@ struct Solver {
     }
     template <int MM>
     void launch_divchi(const TvArgs &ta) {
-        dim3 grid((n + 63) / 64, (n + 3) / 4, nch);
-        launch("divchi", [&] { hipLaunchKernelGGL(k_divchi<MM>, grid, dim3(256), 0, st, d_psi.p, d_mu.p, d_dchi.p, ta); });
+        const dim3 grid(tv_blocks(n, nch));
+        launch("divchi", [&] {
+            hipLaunchKernelGGL(k_divchi<MM>, grid, dim3(TV_THREADS), 0, st, d_psi.p, d_mu.p, d_dchi.p, ta);
+        });
     }
     void divchi(const TvArgs &ta) {
         if (M <= 4) launch_divchi<4>(ta);
