@@ -207,6 +207,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true", help="also run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default gather/scatter launch shapes")
     ap.add_argument("--mode", default="all", choices=["all", "plain", "batched", "served"],
                     help="plain: the headline solve only; batched / served: those forms only; all: each")
     ap.add_argument("--jobs", type=int, default=64, help="jobs in the batched / served mixes")
@@ -238,6 +239,11 @@ def main():
     s = Solver(n, nt, m, nz, nzp, 1)
     s.set_inputs(w["data"], w["theta"], w["phi"], w["center"])
     s.reset()
+    tuned = None
+    if not a.no_autotune:
+        tuned = s.autotune()
+        s.reset()
+        print("bench: autotuned gather/scatter slices per workgroup %s" % (tuned,), file=sys.stderr, flush=True)
     print("bench: solver ready, %.1f GB on device" % (s.device_bytes() / 1e9), file=sys.stderr, flush=True)
     if a.warmup > 0:
         s.iterate(a.warmup, a.titer, lam0, lam1)
@@ -283,7 +289,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain ADMM solve, N=%d Nz=%d Nzp=%d ntheta=%d (Nrot=%d x %d) M=%d titer=%d"
                    % (n, nz, nzp, nt, w["nrot"], w["ntheta_rot"], m, a.titer),
-                   "n": n, "nz": nz, "nzp": nzp, "ntheta": nt, "m": m, "titer": a.titer},
+                   "n": n, "nz": nz, "nzp": nzp, "ntheta": nt, "m": m, "titer": a.titer,
+                   "launch_shapes": None if tuned is None else {"gather_slices": tuned[0], "scatter_slices": tuned[1]}},
         "iteration_hbm_frac": round(B / (ms * 1e-3) / HBM_PEAK, 4),
         "iteration_model_bytes": B,
         "roofline": roof,

@@ -50,6 +50,16 @@ constexpr int FFT_WG = 512;
 __host__ __device__ constexpr int fft_rows_p2(int L) {
     return (FFT_WG / (L / FFT_E)) < 1 ? 1 : ((FFT_WG / (L / FFT_E)) < 16 ? (FFT_WG / (L / FFT_E)) : 16);
 }
+// Column passes: FFT_THREADS / (L / FFT_E) adjacent columns per workgroup (at most
+// 16), so that the strided side of the pass moves 64-byte row segments at L = 2048
+// (one workgroup per CU; LDS ~140 KB).
+#ifndef RTV_COLS_THREADS
+#define RTV_COLS_THREADS 1024  // threads per column-pass workgroup (build-time tuning constant)
+#endif
+__host__ __device__ constexpr int fft_cols_p2(int L) {
+    return (RTV_COLS_THREADS / (L / FFT_E)) < 1 ? 1
+           : ((RTV_COLS_THREADS / (L / FFT_E)) < 16 ? (RTV_COLS_THREADS / (L / FFT_E)) : 16);
+}
 
 struct SampleGeom {
     int row0;   // first window row in the 2n grid (already mod 2n)
@@ -158,7 +168,7 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
     if constexpr (NP > 0) {
         // thread (t, jl), t fastest: adjacent lanes read adjacent columns; the n
         // input rows sit at positions 4*TPT .. 12*TPT - 1
-        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_rows_p2(NP);
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_cols_p2(NP);
         const int ntile = N / TT;
         const int kt = L % ntile;
         const int zj = L / ntile;  // z*MP + p
@@ -898,7 +908,7 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     if constexpr (NP > 0) {
         // thread (t, jl), t fastest; output positions 4*TPT .. 12*TPT - 1 are kept
-        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_rows_p2(NP);
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_cols_p2(NP);
         const int ntile = N / TT;
         const int kt = L % ntile;
         const int zj = L / ntile;
@@ -1018,7 +1028,10 @@ struct TvArgs {
 // The 1D grid (x tiles fastest, then y tiles, then chunks) is remapped so that
 // consecutive tiles share an XCD and its L2, which then serves the rows and
 // columns that neighbouring tiles read across their edges.
-constexpr int TV_ROWS = 16;
+#ifndef RTV_TV_ROWS
+#define RTV_TV_ROWS 16  // rows per TV workgroup (build-time tuning constant)
+#endif
+constexpr int TV_ROWS = RTV_TV_ROWS;
 constexpr int TV_THREADS = 64 * TV_ROWS;
 struct TvTile {
     int x, y, c, b;  // column, row, chunk, logical workgroup index
@@ -1170,21 +1183,43 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 // scale of the relative change used by callers that stop early).  Thread per
 // dual column (chunk, y', x') over all (n+1)^2 positions, ghost entries included.
 constexpr int PROX_W = 4;  // partial sums per workgroup
+// The M+1 dual frames of a column are split over FS threads (frame groups of
+// Q = ceil((MM+1)/FS)); a group holds the Q+1 frames iq0-1 .. iq0+Q-1 it
+// reads (clamped to [0, M-1]) for the current and the next slab.  Workgroup:
+// 64 x positions x FS groups x TV_ROWS/FS rows.
+#ifndef RTV_PROX_FS
+#define RTV_PROX_FS 1  // frame groups per dual column (build-time tuning constant)
+#endif
+template <int MM>
+__host__ __device__ constexpr int prox_fs() { return MM >= 8 ? RTV_PROX_FS : 1; }
 template <int MM>
 __global__ void __launch_bounds__(TV_THREADS)
 k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__restrict__ psi,
        float4 *__restrict__ mu, TvArgs a, double *__restrict__ partial) {
+    constexpr int FS = prox_fs<MM>();
+    constexpr int Q = (MM + 1 + FS - 1) / FS;
+    constexpr int QL = Q + 1;
+    constexpr int ROWS = TV_ROWS / FS;
     __shared__ double red[PROX_W][TV_THREADS / 64];
-    const TvTile tt = tv_tile(a.n + 1, a.n + 1, a.chunk0);
-    const int xq = tt.x, yq = tt.y, c = tt.c;  // dual position (x', y') in [0, n]^2
     const int n = a.n, M = a.M, nzp = a.nzp;
+    // dual position (x', y') in [0, n]^2 of chunk c, frame group g
+    const int gx = (n + 1 + 63) / 64, gy = (n + 1 + ROWS - 1) / ROWS;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int xq = (L % gx) * 64 + (int)(threadIdx.x & 63);
+    const int g = (int)(threadIdx.x >> 6) % FS;
+    const int yq = ((L / gx) % gy) * ROWS + (int)(threadIdx.x >> 6) / FS;
+    const int c = a.chunk0 + L / (gx * gy);
+    const int iq0 = g * Q;
     const float r = *a.rho;
+    const float rinv = 1.0f / r;
+    const float th = a.lam0 / r;
     double sr = 0.0, ss = 0.0, sc = 0.0, sf = 0.0;
     if (xq <= n && yq <= n) {
         const bool flgl = c != 0, flgr = c != a.nchunk - 1;
         const size_t plane = (size_t)n * n, S = (size_t)M * plane;
         const size_t zb = (size_t)c * nzp;
         auto cl = [n](int q) { return q < 0 ? 0 : (q > n - 1 ? n - 1 : q); };
+        auto clm = [M](int q) { return q < 0 ? 0 : (q > M - 1 ? M - 1 : q); };
         const int yc = cl(yq - 1), xc = cl(xq - 1), yn = cl(yq), xn = cl(xq);
         // slab of extension index zl' (E index), returns global slab and whether it is a halo
         auto slab = [&](int zlq, bool &halo) -> size_t {
@@ -1192,20 +1227,19 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
             const int lo = flgl ? -1 : 0, hi = flgr ? nzp : nzp - 1;
             q = q < lo ? lo : (q > hi ? hi : q);
             halo = (q < 0 || q >= nzp);
-            return zb + q;  // q == -1 -> previous chunk's last slab (unsigned wrap avoided: zb >= 1 then)
+            return zb + q;  // q == -1 -> previous chunk's last slab (zb >= 1 then)
         };
-        float cF[MM], cD[MM], nF[MM], nD[MM];
+        // frames held: local j <-> frame clm(iq0 - 1 + j)
+        float cF[QL], cD[QL], nF[QL], nD[QL];
         {
             bool hl;
             const size_t s0 = slab(0, hl);
 #pragma unroll
-            for (int i = 0; i < MM; i++) {
-                if (i < M) {
-                    const size_t o = s0 * S + i * plane + (size_t)yc * n + xc;
-                    const float av = A[o];
-                    cF[i] = hl ? av : fm[o];
-                    cD[i] = hl ? 0.f : av - cF[i];
-                }
+            for (int j = 0; j < QL; j++) {
+                const size_t o = s0 * S + clm(iq0 - 1 + j) * plane + (size_t)yc * n + xc;
+                const float av = A[o];
+                cF[j] = hl ? av : fm[o];
+                cD[j] = hl ? 0.f : av - cF[j];
             }
         }
         for (int zq = 0; zq <= nzp; zq++) {
@@ -1215,32 +1249,31 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
             const size_t ox = sc0 * S + (size_t)yc * n + xn;
             const size_t oy = sc0 * S + (size_t)yn * n + xc;
 #pragma unroll
-            for (int i = 0; i < MM; i++) {
-                if (i < M) {
-                    const size_t on = sn * S + i * plane + (size_t)yc * n + xc;
-                    const float an = A[on];
-                    nF[i] = hn ? an : fm[on];
-                    nD[i] = hn ? 0.f : an - nF[i];
-                }
+            for (int j = 0; j < QL; j++) {
+                const size_t on = sn * S + clm(iq0 - 1 + j) * plane + (size_t)yc * n + xc;
+                const float an = A[on];
+                nF[j] = hn ? an : fm[on];
+                nD[j] = hn ? 0.f : an - nF[j];
             }
 #pragma unroll
-            for (int iq = 0; iq <= MM; iq++) {
+            for (int jq = 0; jq < Q; jq++) {
+                const int iq = iq0 + jq;
                 if (iq <= M) {
-                    // E index iq reads frame clamp(iq-1); iq+1 reads clamp(iq) (iq <= M)
-                    const int ic = iq > 0 ? iq - 1 : 0;
-                    const int it = iq < MM ? iq : MM - 1;
-                    const float e0 = cF[ic];
-                    const float axv = A[ox + ic * plane], ayv = A[oy + ic * plane];
-                    const float xFv = hc ? axv : fm[ox + ic * plane];
-                    const float yFv = hc ? ayv : fm[oy + ic * plane];
+                    // E index iq reads frame clamp(iq-1) (local jq); iq+1 reads clamp(iq)
+                    // (local jq + 1, used for iq < M)
+                    const size_t fo = (size_t)(iq > 0 ? iq - 1 : 0) * plane;
+                    const float e0 = cF[jq];
+                    const float axv = A[ox + fo], ayv = A[oy + fo];
+                    const float xFv = hc ? axv : fm[ox + fo];
+                    const float yFv = hc ? ayv : fm[oy + fo];
                     const float xDv = hc ? 0.f : axv - xFv, yDv = hc ? 0.f : ayv - yFv;
                     const float hx = a.gs * (xFv - e0);
                     const float hy = a.gs * (yFv - e0);
-                    const float ht = a.gt * ((iq < M ? cF[it] : e0) - e0);
-                    const float hz = a.gs * (nF[ic] - e0);
-                    const float d0 = cD[ic];
+                    const float ht = a.gt * ((iq < M ? cF[jq + 1] : e0) - e0);
+                    const float hz = a.gs * (nF[jq] - e0);
+                    const float d0 = cD[jq];
                     const float dx = a.gs * (xDv - d0), dy = a.gs * (yDv - d0);
-                    const float dt = a.gt * ((iq < M ? cD[it] : d0) - d0), dz = a.gs * (nD[ic] - d0);
+                    const float dt = a.gt * ((iq < M ? cD[jq + 1] : d0) - d0), dz = a.gs * (nD[jq] - d0);
                     ss += (double)dx * dx + (double)dy * dy + (double)dt * dt + (double)dz * dz;
                     if (zq >= 1 && iq >= 1 && yq >= 1 && xq >= 1) {
                         sc += (double)d0 * d0;
@@ -1248,16 +1281,16 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
                     }
                     const size_t d = dual_idx(a, c, zq, iq, yq, xq);
                     float4 m4 = mu[d];
-                    float4 p4 = make_float4(hx + m4.x / r, hy + m4.y / r, ht + m4.z / r, hz + m4.w / r);
+                    float4 p4 = make_float4(hx + m4.x * rinv, hy + m4.y * rinv, ht + m4.z * rinv, hz + m4.w * rinv);
                     const float za = sqrtf(p4.x * p4.x + p4.y * p4.y + p4.z * p4.z + p4.w * p4.w);
-                    const float th = a.lam0 / r;
                     if (za <= th) {
                         p4 = make_float4(0.f, 0.f, 0.f, 0.f);
                     } else {
-                        p4.x -= th * p4.x / za;
-                        p4.y -= th * p4.y / za;
-                        p4.z -= th * p4.z / za;
-                        p4.w -= th * p4.w / za;
+                        const float sh = th / za;
+                        p4.x -= sh * p4.x;
+                        p4.y -= sh * p4.y;
+                        p4.z -= sh * p4.z;
+                        p4.w -= sh * p4.w;
                     }
                     m4.x += r * hx - r * p4.x;
                     m4.y += r * hy - r * p4.y;
@@ -1270,11 +1303,9 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
                 }
             }
 #pragma unroll
-            for (int i = 0; i < MM; i++) {
-                if (i < M) {
-                    cF[i] = nF[i];
-                    cD[i] = nD[i];
-                }
+            for (int j = 0; j < QL; j++) {
+                cF[j] = nF[j];
+                cD[j] = nD[j];
             }
         }
     }
@@ -1296,7 +1327,7 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
         for (int q = 0; q < PROX_W; q++) {
             double t = 0.0;
             for (int k = 0; k < TV_THREADS / 64; k++) t += red[q][k];
-            partial[PROX_W * (size_t)tt.b + q] = t;
+            partial[PROX_W * (size_t)L + q] = t;
         }
     }
 }

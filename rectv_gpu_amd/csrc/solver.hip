@@ -543,7 +543,7 @@ struct Solver {
         d_v.alloc((size_t)zb * MP * n * n, bytes);
         d_gq.alloc((size_t)zb * nt * n * 2, bytes);
         d_h1.alloc((size_t)zb * nt * n, bytes);
-        const size_t pb = (size_t)tv_blocks(n + 1, nch);
+        const size_t pb = (size_t)((n + 1 + 63) / 64) * (n + 1) * nch;  // >= prox workgroups
         const size_t rb = (size_t)((zb * nt + T_ray - 1) / T_ray);
         d_part.alloc(PROX_W * std::max(pb, rb) + 16, bytes);
         d_red.alloc(8, bytes);
@@ -592,8 +592,10 @@ struct Solver {
         T_ray = std::min(tn, 16);
         // power-of-two lengths use compile-time row counts inside the kernels
         if ((N & (N - 1)) == 0 && N >= 64 && N <= 4096) {
-            T_cols = T_arows = T_rows = fft_rows_p2(N);
-            RTV_REQUIRE(8 * st_N * T_cols <= lds_max, "LDS too small for the 2n-point transforms");
+            T_arows = T_rows = fft_rows_p2(N);
+            T_cols = fft_cols_p2(N);
+            RTV_REQUIRE(8 * st_N * T_cols <= lds_max && 8 * st_N * T_rows <= lds_max,
+                        "LDS too small for the 2n-point transforms");
         }
         if ((n & (n - 1)) == 0 && n >= 64 && n <= 4096) {
             T_ray = fft_rows_p2(n);
@@ -857,7 +859,8 @@ struct Solver {
     }
     template <int MM>
     void launch_prox(const float *cur, const float *A, const TvArgs &ta) {
-        const dim3 grid(tv_blocks(n + 1, nch));
+        const int rows = TV_ROWS / prox_fs<MM>();
+        const dim3 grid(((n + 1 + 63) / 64) * ((n + 1 + rows - 1) / rows) * nch);
         launch("prox", [&] {
             hipLaunchKernelGGL(k_prox<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
         });
