@@ -419,6 +419,7 @@ struct Solver {
     // FFT
     FftPlan plan_n{}, plan_N{};
     bool odd_n = false, odd_N = false;  // lengths with factors 3, 5, 7
+    bool p2N = false;                   // 2n has a compile-time power-of-two kernel
     DBuf<float2> d_tw;
     int T_rows = 1, T_cols = 1, T_ray = 1, T_arows = 1, st_n = 0, st_N = 0;
     // state
@@ -591,7 +592,8 @@ struct Solver {
         T_arows = T_rows = std::min(tN, 16);
         T_ray = std::min(tn, 16);
         // power-of-two lengths use compile-time row counts inside the kernels
-        if ((N & (N - 1)) == 0 && N >= 64 && N <= 4096) {
+        p2N = (N & (N - 1)) == 0 && N >= 64 && N <= 4096;
+        if (p2N) {
             T_arows = T_rows = fft_rows_p2(N);
             T_cols = fft_cols_p2(N);
             RTV_REQUIRE(8 * st_N * T_cols <= lds_max && 8 * st_N * T_rows <= lds_max,
@@ -786,6 +788,7 @@ struct Solver {
     }
     void scatter(int zc, int zw) {
         if (zw == 2) scatter_t<2>(zc);
+        else if (zw == 8) scatter_t<8>(zc);
         else scatter_t<4>(zc);
     }
 
@@ -1170,7 +1173,7 @@ struct Solver {
             forward_to_rays(d_f[ia].p, zc, d_g.p, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);
             float bestg = 1e30f, bests = 1e30f;
             int zg = GM_ZG, zw = SM_ZW;
-            for (int c : {1, 2, 4, 8, 16}) {
+            for (int c : {1, 2, 4, 8, 16, 32}) {
                 if (c > 1 && c / 2 >= zc) break;
                 const float t = time_best([&] { gather(zc, c); });
                 if (t < bestg) {
@@ -1179,7 +1182,7 @@ struct Solver {
                 }
             }
             ray(zc, d_g.p, nullptr, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);  // q back in gq
-            for (int c : {2, 4}) {
+            for (int c : {2, 4, 8}) {
                 const float t = time_best([&] { scatter(zc, c); });
                 if (t < bests) {
                     bests = t;

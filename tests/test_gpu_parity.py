@@ -44,12 +44,20 @@ def _case(name):
         n, nt, m, nz = 128, 128, 4, 8
         th = B.angles(32, 4)
         phi, center = B.realbasis(nt, m), n / 2
+    elif name == "n1024_column_rounds":   # 2n = 2048: two column rounds per workgroup
+        n, nt, m, nz = 1024, 12, 3, 1
+        th = B.angles(6, 2)
+        phi, center = B.takephi(nt, m), n / 2 - 3.5
+    elif name == "n2048_column_rounds":   # 2n = 4096: 8-column workgroups
+        n, nt, m, nz = 2048, 6, 2, 1
+        th = B.angles(3, 2)
+        phi, center = B.takephi(nt, m), n / 2
     else:
         raise KeyError(name)
     return n, nt, m, nz, th, phi, center, rng
 
 
-CASES = ["n64_rand", "n72_mixed_radix_m3", "cfg_small"]
+CASES = ["n64_rand", "n72_mixed_radix_m3", "cfg_small", "n1024_column_rounds", "n2048_column_rounds"]
 
 
 @pytest.mark.parametrize("name", CASES)
