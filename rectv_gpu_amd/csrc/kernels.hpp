@@ -68,6 +68,15 @@ struct SampleGeom {
     float fx;   // X - floor(X)
 };
 
+// An adjoint list entry: the item (2*sample + mirror) with its window geometry,
+// stored in list order so that staging a chunk reads consecutive records.
+struct AdjItem {
+    int pid;    // item id
+    int rc;     // row0 | col0 << 16
+    float fy;
+    float fx;
+};
+
 struct TileEntry {
     int tile;
     int start;
@@ -739,8 +748,8 @@ constexpr int SM_ZW = 4;    // default slices per workgroup (template argument Z
 constexpr int SM_LB = 16 * (SM_CH + 4);  // list bytes per chunk (16 blocks, padded)
 template <int NB, int ZW>   // 16-column blocks: MP <= 8*NB (NB = 1 or 2); ZW slices per workgroup
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))  // two workgroups per CU
-k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
-          const int *__restrict__ sids, const TileEntry *__restrict__ tiles, const int2 *__restrict__ crange,
+k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *__restrict__ aitems,
+          const TileEntry *__restrict__ tiles, const int2 *__restrict__ crange,
           const int2 *__restrict__ brange, const unsigned char *__restrict__ blist, const float2 *__restrict__ cf,
           int n, int nt, int MP, float beta, int ntiles, int nzb) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
@@ -780,7 +789,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
     // chunk staging data are prefetched into registers one chunk ahead, so the
     // dependent global loads (item -> geometry, q, coefficients) overlap the MFMA loop
     int pid = -1;
-    SampleGeom pg = {0, 0, 0.f, 0.f};
+    AdjItem pg = {0, 0, 0.f, 0.f};
     float2 pq[ZW], pph[NPP];
     constexpr int NLB = (SM_LB + 4 * 1024 - 1) / (4 * 1024);  // list words per thread
     unsigned plw[NLB];
@@ -789,8 +798,8 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
         const int cnt = min(SM_CH, te.count - c0);
         pid = -1;
         if (sm < cnt) {
-            pid = sids[te.start + c0 + sm];
-            pg = geo[pid];
+            pg = aitems[te.start + c0 + sm];
+            pid = pg.pid;
             const int sid = pid >> 1;
             const bool mir = pid & 1;
 #pragma unroll
@@ -826,10 +835,10 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const SampleGeo
             if (4 * wi < pcr.y) reinterpret_cast<unsigned *>(sl)[wi] = plw[q];
         }
         if (pid >= 0) {
-            const SampleGeom g = pg;
-            int dy = g.row0 - ty0;
+            const AdjItem g = pg;
+            int dy = (g.rc & 0xffff) - ty0;
             if (dy < 0) dy += N;
-            int dx = g.col0 - tx0;
+            int dx = (g.rc >> 16) - tx0;
             if (dx < 0) dx += N;
             const int ly0 = dy < TS ? dy : dy - N;
             const int lx0 = dx < TS ? dx : dx - N;

@@ -65,6 +65,7 @@ struct HostGeom {
     std::vector<int4> fgroups;
     std::vector<int> fitems;  // forward items per origin tile for the matrix-core gather (-1 = padding)
     std::vector<int> asids;
+    std::vector<AdjItem> aitems;  // asids with their window geometry
     std::vector<TileEntry> ftiles, fmtiles, atiles;  // atiles[].pad: first chunk of the tile
     // adjoint chunks (SM_CH items of a tile): crange = (byte offset, byte length) of the
     // chunk's block lists in blist; brange per (chunk, 4x4 block) = (offset within the
@@ -280,6 +281,11 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             cnt[keys[a].k >> 32]++;
         }
         for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
+        hg.aitems.resize(hg.asids.size());
+        for (size_t a = 0; a < hg.asids.size(); a++) {
+            const SampleGeom &g = hg.geo[hg.asids[a]];
+            hg.aitems[a] = AdjItem{hg.asids[a], g.row0 | (g.col0 << 16), g.fy, g.fx};
+        }
         hg.atiles.resize(ntile);
         for (int t = 0; t < ntile; t++) hg.atiles[t] = TileEntry{t, off[t], cnt[t], 0};
         // per chunk of SM_CH items and 4 x 4 cell block: the slots of the items whose
@@ -452,7 +458,7 @@ struct Solver {
     DBuf<SampleGeom> d_geo;
     DBuf<int4> d_fgroups;
     DBuf<int> d_fitems;
-    DBuf<int> d_asids;
+    DBuf<AdjItem> d_aitems;
     DBuf<TileEntry> d_ftiles, d_fmtiles, d_atiles;
     DBuf<int2> d_crange, d_brange;
     DBuf<unsigned char> d_blist;
@@ -496,11 +502,11 @@ struct Solver {
         const size_t state = 3 * V * 4 + 2 * V4 * 16 + V * 4 + (size_t)nz * nt * n * 4;
         const size_t per_slice = MP * n * N * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 16 + (size_t)nt * n * 4;
         const size_t ni = 2 * (size_t)nt * n, ntile = (N / TS) * (N / TS);
-        // geo 16 B, VALU groups <= 16 B, gather slots <= 16 ints, adjoint lists <= 4 ints and
-        // block lists <= 9 bytes per item; per adjoint chunk its range, 16 block ranges and
-        // up to 3 padding bytes per block; tables per ray / pixel
+        // geo 16 B, VALU groups <= 16 B, gather slots <= 16 ints, adjoint lists <= 4 records
+        // of 16 B and block lists <= 9 bytes per item; per adjoint chunk its range, 16 block
+        // ranges and up to 3 padding bytes per block; tables per ray / pixel
         const size_t nchk = ntile + 4 * ni / SM_CH + 1;
-        const size_t geom = ni * (16 + 16 + 64 + 16 + 9) + nchk * (8 + 16 * 8 + 16 * 3) + 3 * ntile * 16 +
+        const size_t geom = ni * (16 + 16 + 64 + 64 + 9) + nchk * (8 + 16 * 8 + 16 * 3) + 3 * ntile * 16 +
                             (size_t)nt * 2 * MP * 8 + 8 * (size_t)n * 8 + 2 * (N + n) * 8;
         return state + per_slice * nz + geom + (1 << 20);
     }
@@ -672,7 +678,7 @@ struct Solver {
             fprintf(stderr, "rectv: gather items %zu in %zu slots, %d entries\n", real, hg.fitems.size(), nfm_entries);
         }
         gather_valu = std::getenv("RECTV_GATHER") && std::string(std::getenv("RECTV_GATHER")) == "valu";
-        upload(d_asids, hg.asids, bytes, st);
+        upload(d_aitems, hg.aitems, bytes, st);
         upload(d_atiles, hg.atiles, bytes, st);
         upload(d_crange, hg.crange, bytes, st);
         upload(d_brange, hg.brange, bytes, st);
@@ -777,12 +783,12 @@ struct Solver {
         launch("scatter", [&] {
             // MP <= 16 (M <= 32): one or two 16-column blocks
             if (MP <= 8)
-                hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                   d_asids.p, d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, n, nt, MP, beta,
+                hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_aitems.p,
+                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, n, nt, MP, beta,
                                    na_tiles, zc);
             else
-                hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                   d_asids.p, d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, n, nt, MP, beta,
+                hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_aitems.p,
+                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, n, nt, MP, beta,
                                    na_tiles, zc);
         });
     }
@@ -1257,7 +1263,7 @@ struct Solver {
         d_red.release(t);
         d_geo.release(t);
         d_fgroups.release(t);
-        d_asids.release(t);
+        d_aitems.release(t);
         d_ftiles.release(t);
         d_fitems.release(t);
         d_fmtiles.release(t);
