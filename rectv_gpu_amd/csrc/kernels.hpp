@@ -396,35 +396,36 @@ k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom
 }
 
 // F3 on the matrix cores.  One workgroup per (tile entry of <= GM_ITEMS items
-// whose window origin lies in the tile, group of GM_ZG slices); items are sorted
+// whose window origin lies in the tile, group of zg slices); items are sorted
 // by local origin (row, column) and packed by the host into 16-item blocks (one
-// wave each) spanning at most 12 rows and 4 steps of 4 cells.  Item weights are
-// staged once as 24-wide rows that are zero outside the window; per slice the
-// tile plus its 8-cell halo of all MP pair spectra is staged as
-// tile[row][cell][col], col = p (real part) and HALF + p (imaginary part), the
-// next slice's tile being loaded into registers meanwhile.  Per window row r of
-// its block and per 4 cells, a wave runs
-//   D[col][item] += sum_c tile[r][c][col] * wyr[item][r] * wxr[item][c]
+// wave each) spanning at most 12 rows and 4 steps of 4 cells.  The 9 window
+// taps per axis of every item are staged once; per slice the tile plus its
+// 8-cell halo of all MP pair spectra is staged as tile[row][cell][col], col = p
+// (real part) and HALF + p (imaginary part), the next slice's tile being loaded
+// into registers meanwhile.  Per window row r of its block and per 4 cells, a
+// wave runs
+//   D[col][item] += sum_c tile[r][c][col] * wy[item][r] * wx[item][c]
 // as v_mfma_f32_16x16x4f32 (16 columns x 16 items x 4 cells); each lane then
 // holds 4 columns of one item, contracts them with cf (alpha_p[k] direct,
 // conj(beta_p[k]) mirror) and two cross-lane steps finish the sum over pairs.
-constexpr int GM_ITEMS = 128;  // items per workgroup: 8 waves x 16
+constexpr int GM_ITEMS = 256;  // items per workgroup: 16 waves x 16
+constexpr int GM_THREADS = 1024;
 constexpr int GM_KS = 4;       // max steps of 4 cells per block (the host packs blocks so)
 constexpr int GM_ZG = 8;       // default slices per workgroup (runtime argument zg)
 template <int NB>              // 16-column blocks: MP <= 8*NB
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(GM_THREADS)
 k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
               const int *__restrict__ items, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf,
               int n, int nt, int MP, float beta, int nentries, int nzb, int zg) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
     constexpr int NC = 16 * NB, HALF = 8 * NB, NCP = NC + 1;
     constexpr int CELLS = TH * TH;
-    constexpr int NLD = (HALF * CELLS + 511) / 512;  // tile loads per thread and slice
+    constexpr int NLD = (HALF * CELLS + GM_THREADS - 1) / GM_THREADS;  // tile loads per thread and slice
     // 4 spare cells: the last 4-cell step of row TH-1 may read up to 3 cells past
     // the region (their weights are zero; the spare cells are zeroed)
     __shared__ float tile[(CELLS + 4) * NCP];
-    __shared__ float wyr[GM_ITEMS][TH];
-    __shared__ float wxr[GM_ITEMS][TH];
+    __shared__ float wty[GM_ITEMS][KW];  // the 9 window taps along rows / cells
+    __shared__ float wtx[GM_ITEMS][KW];
     __shared__ int sid[GM_ITEMS];
     __shared__ int sly[GM_ITEMS], slx[GM_ITEMS];
     const int tid = threadIdx.x;
@@ -441,7 +442,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     int tofs[NLD];
 #pragma unroll
     for (int q = 0; q < NLD; q++) {
-        const int idx = tid + q * 512;
+        const int idx = tid + q * GM_THREADS;
         const int p = idx / CELLS, rc = idx - p * CELLS;
         const int r = rc / TH, c = rc - r * TH;
         int gy = ty0 + r;
@@ -459,7 +460,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     auto store_tile = [&] {
 #pragma unroll
         for (int q = 0; q < NLD; q++) {
-            const int idx = tid + q * 512;
+            const int idx = tid + q * GM_THREADS;
             if (idx < HALF * CELLS) {
                 const int p = idx / CELLS, rc = idx - p * CELLS;
                 tile[rc * NCP + p] = stv[q].x;
@@ -468,7 +469,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
         }
     };
     load_tile(z0);
-    // stage the items: 4 threads per item, 6 of the 24 weight entries each
+    // stage the items: 4 threads per item (rows taps 0-4 / 5-8, cell taps 0-4 / 5-8)
     {
         const int it = tid >> 2, part = tid & 3;
         const int id = it < cnt ? items[te.start + it] : -1;
@@ -486,14 +487,10 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
             sly[it] = ly0;
             slx[it] = lx0;
         }
-#pragma unroll
-        for (int h = 0; h < TH / 4; h++) {
-            const int r = part * (TH / 4) + h;
-            const int ay = r - ly0, ax = r - lx0;
-            const float dyv = (float)(ay - 4) - g.fy, dxv = (float)(ax - 4) - g.fx;
-            wyr[it][r] = ((unsigned)ay < (unsigned)KW) ? gauss(beta, dyv) : 0.f;
-            wxr[it][r] = ((unsigned)ax < (unsigned)KW) ? gauss(beta, dxv) : 0.f;
-        }
+        const float fr = (part < 2) ? g.fy : g.fx;
+        float *wt = (part < 2) ? &wty[it][0] : &wtx[it][0];
+        const int a0 = (part & 1) ? 5 : 0, a1 = (part & 1) ? KW : 5;
+        for (int a = a0; a < a1; a++) wt[a] = gauss(beta, (float)(a - 4) - fr);
         if (tid < 4 * NCP) tile[CELLS * NCP + tid] = 0.f;
     }
     __syncthreads();
@@ -532,9 +529,10 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     float wxv[GM_KS];
 #pragma unroll
     for (int ks = 0; ks < GM_KS; ks++) {
-        const int c = clo + 4 * ks + lk;
-        wxv[ks] = (ks < nks && c < TH && rhi >= 0) ? wxr[b0 + li][c] : 0.f;
+        const int ax = clo + 4 * ks + lk - slx[b0 + li];
+        wxv[ks] = (ks < nks && (unsigned)ax < (unsigned)KW && rhi >= 0) ? wtx[b0 + li][ax] : 0.f;
     }
+    const int myly = sly[b0 + li];
     float2 *out = gq + (size_t)z0 * nt * n * 2;
     for (int zz = 0; zz < zc; zz++) {
         if (zz > 0) __syncthreads();  // previous slice's tile fully read
@@ -547,7 +545,8 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
             for (int b = 0; b < NB; b++) acc0[b] = acc1[b] = (v4f){0.f, 0.f, 0.f, 0.f};
             // a 9-cell window spans exactly 3 steps of 4 cells; blocks span 3 or 4
             for (int r = rlo; r <= rhi; r++) {
-                const float wy = wyr[b0 + li][r];
+                const int ay = r - myly;
+                const float wy = (unsigned)ay < (unsigned)KW ? wty[b0 + li][ay] : 0.f;
                 const float *trow = tile + (r * TH + clo + lk) * NCP + li;
                 float bv[GM_KS][NB];
 #pragma unroll

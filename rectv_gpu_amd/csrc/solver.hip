@@ -769,10 +769,10 @@ struct Solver {
         if (blocks > 0)
             launch("gather", [&] {
                 if (MP <= 8)
-                    hipLaunchKernelGGL(k_gather_mfma<1>, dim3(blocks), dim3(512), 0, st, d_G.p, d_gq.p, d_geo.p,
+                    hipLaunchKernelGGL(k_gather_mfma<1>, dim3(blocks), dim3(GM_THREADS), 0, st, d_G.p, d_gq.p, d_geo.p,
                                        d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc, zg);
                 else
-                    hipLaunchKernelGGL(k_gather_mfma<2>, dim3(blocks), dim3(512), 0, st, d_G.p, d_gq.p, d_geo.p,
+                    hipLaunchKernelGGL(k_gather_mfma<2>, dim3(blocks), dim3(GM_THREADS), 0, st, d_G.p, d_gq.p, d_geo.p,
                                        d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc, zg);
             });
     }
