@@ -36,7 +36,6 @@ constexpr int FFT_THREADS = 1024;
 constexpr int TS = 16;          // spectral tile edge (cells) for gather/scatter
 constexpr int KW = 9;           // Gaussian window taps per axis, 2K+1 with K = 4
 constexpr int TH = TS + KW - 1; // tile + halo rows loaded by the gather
-constexpr int TP = TH + 1;      // LDS row pitch of the gather tile
 
 __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 
@@ -230,172 +229,15 @@ __device__ __forceinline__ void window_weights(float frac, float beta, float *w)
     for (int a = 0; a < KW; a++) w[a] = gauss(beta, (float)(a - 4) - frac);
 }
 
-// F3: tile-binned Gaussian gather from the MP pair spectra of slice z (replaces
-// gather FWD kernels_radonusfft.cuh:81-144 run once per basis function, plus
-// mulphi/addreal's sum over j, operators.cu:22-31).
+// F3: tile-binned Gaussian gather from the MP pair spectra on the matrix cores
+// (replaces gather FWD, kernels_radonusfft.cuh:81-144, run once per basis
+// function, plus mulphi/addreal's sum over j, operators.cu:22-31).
 // Items: sample (k, s) has a direct item 2*(k*n+s) (its own window) and a mirror
 // item 2*(k*n+s)+1 (the point-mirrored window, fractions negated).  Item sums
 // are contracted with cf[k][kind][p] (alpha_p[k] direct, conj(beta_p[k]) mirror)
 // and written to gq[z][item]; the per-ray kernel forms direct + conj(mirror).
-// Work unit: a group of up to GSZ items sharing one 9x9 window origin (rays
-// repeated every 2 pi, and rays theta+pi read in reverse detector order, land on
-// the same frequency cells).  One workgroup per (tile entry of <= GMAX groups,
-// slice z).  The group weights are staged once in LDS; the spectra tile plus its
-// 8-cell halo is staged GJC spectra at a time; lane = (group, p) with p fastest,
-// so each lane reads the window taps of one spectrum once for all group
-// members, and the contraction over p is a GJC-lane shuffle reduction.
-constexpr int GMAX = 64;     // groups per gather workgroup
-constexpr int GJC = 4;       // spectra staged per pass (power of two)
-template <int GSZ>
-__global__ void __launch_bounds__(256)
-k_gather(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
-         const int4 *__restrict__ groups, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf,
-         int n, int nt, int MP, float beta, int nentries) {
-    __shared__ float2 tile[GJC][TH][TP];
-    __shared__ float wxs[GMAX][GSZ][KW];  // column weights per member
-    __shared__ float wys[GMAX][KW][GSZ];  // row weights, members contiguous per row
-    __shared__ float2 acc[GMAX][GSZ];
-    __shared__ int gofs[GMAX];
-    __shared__ int gid[GMAX][GSZ];
-    __shared__ int gcf[GMAX][GSZ];  // coefficient row (k*2 + kind)*MP of each member
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int N = 2 * n, nT = N / TS;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int e = L % nentries;
-    const int z = L / nentries;
-    const TileEntry te = tiles[e];
-    const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
-    const int cnt = te.count;
-    const float2 *Gz = G + (size_t)z * MP * N * N;
-    // stage group geometry and separable weights
-    for (int t = tid; t < cnt * GSZ; t += nthr) {
-        const int g = t / GSZ, q = t - g * GSZ;
-        const int4 gr = groups[te.start + g];
-        const int id = q == 0 ? gr.x : q == 1 ? gr.y : q == 2 ? gr.z : gr.w;
-        gid[g][q] = id;
-        gcf[g][q] = id >= 0 ? ((id >> 1) / n * 2 + (id & 1)) * MP : 0;
-        acc[g][q] = make_float2(0.f, 0.f);
-        if (q == 0) {
-            const SampleGeom g0 = geo[gr.x];
-            gofs[g] = (g0.row0 - ty0) * TP + (g0.col0 - tx0);
-        }
-        if (id >= 0) {
-            const SampleGeom gs = geo[id];
-            float w[KW];
-            window_weights(gs.fy, beta, w);
-#pragma unroll
-            for (int a = 0; a < KW; a++) wys[g][a][q] = w[a];
-            window_weights(gs.fx, beta, w);
-#pragma unroll
-            for (int a = 0; a < KW; a++) wxs[g][q][a] = w[a];
-        } else {
-#pragma unroll
-            for (int a = 0; a < KW; a++) {
-                wys[g][a][q] = 0.f;
-                wxs[g][q][a] = 0.f;
-            }
-        }
-    }
-    for (int jc = 0; jc < MP; jc += GJC) {
-        const int Jc = min(GJC, MP - jc);
-        // stage the tile: every global load of a thread is issued before its LDS
-        // stores (fixed trip count, unrolled) so the memory latency is paid once
-        constexpr int NLD = (GJC * TH * TH + 255) / 256;
-        float2 stv[NLD];
-#pragma unroll
-        for (int q = 0; q < NLD; q++) {
-            const int idx = tid + q * 256;
-            const int jl = idx / (TH * TH);
-            const int rc = idx - jl * TH * TH;
-            const int r = rc / TH, c = rc - r * TH;
-            int gy = ty0 + r;
-            if (gy >= N) gy -= N;
-            int gx = tx0 + c;
-            if (gx >= N) gx -= N;
-            stv[q] = (idx < GJC * TH * TH && jl < Jc) ? Gz[((size_t)(jc + jl) * N + gy) * N + gx]
-                                                      : make_float2(0.f, 0.f);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < NLD; q++) {
-            const int idx = tid + q * 256;
-            if (idx < GJC * TH * TH) {
-                const int jl = idx / (TH * TH);
-                const int rc = idx - jl * TH * TH;
-                tile[jl][rc / TH][rc - (rc / TH) * TH] = stv[q];
-            }
-        }
-        __syncthreads();
-        // items (g, jl), jl fastest; GJC divides 64 so a group's lanes share a wave
-        for (int it0 = 0; it0 < cnt * GJC; it0 += nthr) {
-            const int it = it0 + tid;
-            const int g = it / GJC, jl = it - (it / GJC) * GJC;
-            const bool live = it < cnt * GJC && jl < Jc;
-            float2 c[GSZ], cfv[GSZ];
-#pragma unroll
-            for (int q = 0; q < GSZ; q++) c[q] = make_float2(0.f, 0.f);
-            if (live) {
-                // coefficients first: their loads overlap the window sums
-#pragma unroll
-                for (int q = 0; q < GSZ; q++) cfv[q] = cf[gcf[g][q] + jc + jl];
-                float wx[GSZ][KW];
-#pragma unroll
-                for (int q = 0; q < GSZ; q++)
-#pragma unroll
-                    for (int b = 0; b < KW; b++) wx[q][b] = wxs[g][q][b];
-                const float2 *t0 = &tile[jl][0][0] + gofs[g];
-                float2 sum[GSZ];
-#pragma unroll
-                for (int q = 0; q < GSZ; q++) sum[q] = make_float2(0.f, 0.f);
-                // one window row per iteration keeps 9 taps live
-#pragma unroll 1
-                for (int a = 0; a < KW; a++) {
-                    float2 tap[KW];
-#pragma unroll
-                    for (int b = 0; b < KW; b++) tap[b] = t0[a * TP + b];
-                    float wy[GSZ];
-#pragma unroll
-                    for (int q = 0; q < GSZ; q++) wy[q] = wys[g][a][q];
-#pragma unroll
-                    for (int q = 0; q < GSZ; q++) {
-                        float2 rs = make_float2(0.f, 0.f);
-#pragma unroll
-                        for (int b = 0; b < KW; b++) {
-                            rs.x += wx[q][b] * tap[b].x;
-                            rs.y += wx[q][b] * tap[b].y;
-                        }
-                        sum[q].x += wy[q] * rs.x;
-                        sum[q].y += wy[q] * rs.y;
-                    }
-                }
-                // empty member slots have zero weights (sum = 0) and coefficient row 0
-#pragma unroll
-                for (int q = 0; q < GSZ; q++) c[q] = cmul(cfv[q], sum[q]);
-            }
-            // sum the GJC basis functions of each group (lanes g*GJC .. g*GJC+GJC-1)
-#pragma unroll
-            for (int o = GJC / 2; o > 0; o >>= 1)
-#pragma unroll
-                for (int q = 0; q < GSZ; q++) {
-                    c[q].x += __shfl_xor(c[q].x, o, 64);
-                    c[q].y += __shfl_xor(c[q].y, o, 64);
-                }
-            if (it < cnt * GJC && jl == 0) {
-#pragma unroll
-                for (int q = 0; q < GSZ; q++) acc[g][q] = cadd(acc[g][q], c[q]);
-            }
-        }
-    }
-    __syncthreads();
-    float2 *out = gq + (size_t)z * nt * n * 2;
-    for (int t = tid; t < cnt * GSZ; t += nthr) {
-        const int g = t / GSZ, q = t - g * GSZ;
-        const int id = gid[g][q];
-        if (id >= 0) out[id] = acc[g][q];
-    }
-}
-
-// F3 on the matrix cores.  One workgroup per (tile entry of <= GM_ITEMS items
+// Rays repeated every 2 pi, and rays theta+pi read in reverse detector order,
+// share window origins.  One workgroup per (tile entry of <= GM_ITEMS items
 // whose window origin lies in the tile, group of zg slices); items are sorted
 // by local origin (row, column) and packed by the host into 16-item blocks (one
 // wave each) spanning at most 12 rows and 4 steps of 4 cells.  The 9 window
@@ -719,7 +561,6 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
     }
 }
 
-constexpr int GS = 4;  // sample slots per origin group (int4 group records)
 
 // A1: tile-owned Gaussian scatter on the FP32 matrix cores (replaces gather ADJ
 // with global atomicAdd, kernels_radonusfft.cuh:126-131, and wrap ADJ, :59-62,

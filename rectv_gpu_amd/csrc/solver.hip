@@ -62,11 +62,10 @@ static bool make_plan(int N, FftPlan &p) {
 // ---------------------------------------------------------------- geometry
 struct HostGeom {
     std::vector<SampleGeom> geo;
-    std::vector<int4> fgroups;
     std::vector<int> fitems;  // forward items per origin tile for the matrix-core gather (-1 = padding)
     std::vector<int> asids;
     std::vector<AdjItem> aitems;  // asids with their window geometry
-    std::vector<TileEntry> ftiles, fmtiles, atiles;  // atiles[].pad: first chunk of the tile
+    std::vector<TileEntry> fmtiles, atiles;  // atiles[].pad: first chunk of the tile
     // adjoint chunks (SM_CH items of a tile): crange = (byte offset, byte length) of the
     // chunk's block lists in blist; brange per (chunk, 4x4 block) = (offset within the
     // chunk's lists, length): slots of the items whose window meets the block
@@ -75,7 +74,6 @@ struct HostGeom {
     std::vector<float2> pre_f, post_a, cf;
     std::vector<float> post_f, pre_a, deap;
     float beta = 0.f;
-    int fgs = 4;  // items per gather group
 };
 
 // USFFT constants, radonusfft.cu:10-12: mu = -log(eps)/(2 n^2) held in float.
@@ -88,8 +86,7 @@ static int usfft_K(int n, float mu) {
     return (int)std::ceil(2.0 * n * 1 / PI_REF * std::sqrt(-m * std::log(eps) + (m * n) * (m * n) / 4));
 }
 
-static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *theta, const float *phi, float center,
-                           int chunk_max) {
+static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *theta, const float *phi, float center) {
     const int N = 2 * n, nT = N / TS, K = 4;
     const float mu = usfft_mu(n);
     if (usfft_K(n, mu) != K) throw Error(-1, "USFFT half-width differs from 4 for this n");
@@ -130,49 +127,16 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             ftile[2 * id + 1] = (m.row0 / TS) * nT + m.col0 / TS;
         }
     }
-    // Work lists.  Items are grouped by window origin (up to GS per group, ids
-    // ascending) within each tile; forward: the tile holding the origin (split
-    // into entries of at most chunk_max groups); adjoint: every tile the 9x9
-    // window intersects (one entry per tile, empty tiles included).
+    // Work lists.  Forward: the items of the tile holding their window origin, in
+    // origin raster order (ids ascending), packed into 16-item blocks and entries
+    // of at most GM_ITEMS items; adjoint: every tile the 9x9 window intersects
+    // (one entry per tile, empty tiles included).
     struct Key {
         unsigned long long k;  // (tile << 32) | origin
         int id;
         bool operator<(const Key &o) const { return k != o.k ? k < o.k : id < o.id; }
     };
     const int ntile = nT * nT;
-    auto make_groups = [&](std::vector<Key> &keys, std::vector<int4> &grp, std::vector<TileEntry> &ent,
-                           bool all_tiles, int split, int gsz) {
-        std::sort(keys.begin(), keys.end());
-        grp.clear();
-        ent.clear();
-        std::vector<int> gstart(ntile + 1, 0);
-        size_t i = 0;
-        for (int t = 0; t < ntile; t++) {
-            gstart[t] = (int)grp.size();
-            while (i < keys.size() && (int)(keys[i].k >> 32) == t) {
-                size_t j = i;
-                while (j < keys.size() && keys[j].k == keys[i].k) j++;
-                for (size_t a = i; a < j; a += gsz) {
-                    int v[GS];
-                    for (int q = 0; q < GS; q++) v[q] = (q < gsz && a + q < j) ? keys[a + q].id : -1;
-                    grp.push_back(make_int4(v[0], v[1], v[2], v[3]));
-                }
-                i = j;
-            }
-        }
-        gstart[ntile] = (int)grp.size();
-        for (int t = 0; t < ntile; t++) {
-            const int c = gstart[t + 1] - gstart[t];
-            if (c == 0) {
-                if (all_tiles) ent.push_back(TileEntry{t, gstart[t], 0, 0});
-                continue;
-            }
-            for (int s0 = 0; s0 < c; s0 += split)
-                ent.push_back(TileEntry{t, gstart[t] + s0, std::min(split, c - s0), 0});
-        }
-        if (std::getenv("RECTV_DENSE_FIRST"))
-            std::stable_sort(ent.begin(), ent.end(), [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
-    };
     {
         std::vector<Key> keys;
         keys.reserve(ni);
@@ -181,15 +145,7 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             const SampleGeom &g = hg.geo[id];
             keys.push_back(Key{((unsigned long long)ftile[id] << 32) | (unsigned)(g.row0 * N + g.col0), (int)id});
         }
-        // group size for the gather: 4 when windows are shared widely (repeated
-        // rotations), else smaller so that empty slots do not waste work
         std::sort(keys.begin(), keys.end());
-        size_t origins = 0;
-        for (size_t a = 0; a < keys.size(); a++)
-            if (a == 0 || keys[a].k != keys[a - 1].k) origins++;
-        const double share = origins ? (double)keys.size() / origins : 1.0;
-        hg.fgs = share >= 3.0 ? 4 : (share >= 1.6 ? 2 : 1);
-        make_groups(keys, hg.fgroups, hg.ftiles, false, chunk_max, hg.fgs);
         // flat lists for the matrix-core gather: items in origin raster order,
         // packed into 16-item blocks (one wave each); a block is closed early
         // when the next origin run would widen its window beyond 12 rows or 4
@@ -322,9 +278,6 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
                 hg.crange.push_back(make_int2(base, len));
             }
         }
-        if (std::getenv("RECTV_DENSE_FIRST"))
-            std::stable_sort(hg.atiles.begin(), hg.atiles.end(),
-                             [](const TileEntry &a, const TileEntry &b) { return a.count > b.count; });
     }
     // per-ray tables (takeshift kernels_radonusfft.cuh:191-198 with c = -/+(center - n/2),
     // fftshift signs, 1/n, PI/mu, 1/sqrt(n ntheta), ntheta/sqrt(M))
@@ -456,16 +409,14 @@ struct Solver {
     // geometry
     bool inputs = false;
     DBuf<SampleGeom> d_geo;
-    DBuf<int4> d_fgroups;
     DBuf<int> d_fitems;
     DBuf<AdjItem> d_aitems;
-    DBuf<TileEntry> d_ftiles, d_fmtiles, d_atiles;
+    DBuf<TileEntry> d_fmtiles, d_atiles;
     DBuf<int2> d_crange, d_brange;
     DBuf<unsigned char> d_blist;
     DBuf<float2> d_cf, d_pre_f, d_post_a;
     DBuf<float> d_post_f, d_pre_a, d_deap;
-    int nf_entries = 0, nfm_entries = 0, na_tiles = 0, fgs = 4;
-    bool gather_valu = false;  // RECTV_GATHER=valu selects the VALU gather
+    int nfm_entries = 0, na_tiles = 0;
     float beta = 0.f;
     // profiling
     bool prof = false;
@@ -663,12 +614,9 @@ struct Solver {
         RTV_REQUIRE(std::isfinite(center), "center must be finite");
         drop_graphs();  // the geometry buffers below are reallocated
         HostGeom hg;
-        build_geometry(hg, n, nt, M, theta, phi, center, GMAX);
-        fgs = hg.fgs;
+        build_geometry(hg, n, nt, M, theta, phi, center);
         beta = hg.beta;
         upload(d_geo, hg.geo, bytes, st);
-        upload(d_fgroups, hg.fgroups, bytes, st);
-        upload(d_ftiles, hg.ftiles, bytes, st);
         upload(d_fitems, hg.fitems, bytes, st);
         upload(d_fmtiles, hg.fmtiles, bytes, st);
         nfm_entries = (int)hg.fmtiles.size();
@@ -677,7 +625,6 @@ struct Solver {
             for (int v : hg.fitems) real += v >= 0;
             fprintf(stderr, "rectv: gather items %zu in %zu slots, %d entries\n", real, hg.fitems.size(), nfm_entries);
         }
-        gather_valu = std::getenv("RECTV_GATHER") && std::string(std::getenv("RECTV_GATHER")) == "valu";
         upload(d_aitems, hg.aitems, bytes, st);
         upload(d_atiles, hg.atiles, bytes, st);
         upload(d_crange, hg.crange, bytes, st);
@@ -689,7 +636,6 @@ struct Solver {
         upload(d_pre_a, hg.pre_a, bytes, st);
         upload(d_post_a, hg.post_a, bytes, st);
         upload(d_deap, hg.deap, bytes, st);
-        nf_entries = (int)hg.ftiles.size();
         na_tiles = (int)hg.atiles.size();
         RTV_HIP(hipMemcpyAsync(d_g.p, g, (size_t)nz * nt * n * sizeof(float), hipMemcpyHostToDevice, st));
         RTV_HIP(hipStreamSynchronize(st));
@@ -742,25 +688,7 @@ struct Solver {
                                    T_cols, st_N);
             });
         }
-        {
-            if (!gather_valu) {
-                gather(zc, gather_zg);
-            } else {
-            const int blocks = nf_entries * zc;
-            if (blocks > 0)
-                launch("gather", [&] {
-                    if (fgs == 4)
-                        hipLaunchKernelGGL(k_gather<4>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                           d_fgroups.p, d_ftiles.p, d_cf.p, n, nt, MP, beta, nf_entries);
-                    else if (fgs == 2)
-                        hipLaunchKernelGGL(k_gather<2>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                           d_fgroups.p, d_ftiles.p, d_cf.p, n, nt, MP, beta, nf_entries);
-                    else
-                        hipLaunchKernelGGL(k_gather<1>, dim3(blocks), dim3(256), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                           d_fgroups.p, d_ftiles.p, d_cf.p, n, nt, MP, beta, nf_entries);
-                });
-            }
-        }
+        gather(zc, gather_zg);
         ray(zc, gdat, nullptr, h1out, mode, partial);
     }
 
@@ -1262,9 +1190,7 @@ struct Solver {
         d_part.release(t);
         d_red.release(t);
         d_geo.release(t);
-        d_fgroups.release(t);
         d_aitems.release(t);
-        d_ftiles.release(t);
         d_fitems.release(t);
         d_fmtiles.release(t);
         d_atiles.release(t);
