@@ -47,6 +47,17 @@ __device__ inline int xcd_remap(int p, int total) {
     return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also waits
+// for every outstanding global load and store of the wave (vmcnt(0) on this
+// architecture), which would stall on data that was prefetched into registers
+// for a later phase; the LDS-scoped fences wait for LDS operations (lgkmcnt)
+// alone.  For kernels whose waves exchange data through LDS only.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Raw buffer access: a wave-uniform base in a buffer descriptor (SGPRs), a
 // 32-bit per-lane byte offset and a wave-uniform byte offset.  Keeps per-lane
 // address state at one VGPR where a loop walks many uniform sub-arrays.

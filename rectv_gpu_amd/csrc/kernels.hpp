@@ -335,7 +335,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
         for (int a = a0; a < a1; a++) wt[a] = gauss(beta, (float)(a - 4) - fr);
         if (tid < 4 * NCP) tile[CELLS * NCP + tid] = 0.f;
     }
-    __syncthreads();
+    lds_barrier();
     const int w = tid >> 6, l = tid & 63;
     const int li = l & 15, lk = l >> 4;
     const int b0 = w * 16;
@@ -377,10 +377,10 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     const int myly = sly[b0 + li];
     float2 *out = gq + (size_t)z0 * nt * n * 2;
     for (int zz = 0; zz < zc; zz++) {
-        if (zz > 0) __syncthreads();  // previous slice's tile fully read
+        if (zz > 0) lds_barrier();  // previous slice's tile fully read
         store_tile();
         if (zz + 1 < zc) load_tile(z0 + zz + 1);
-        __syncthreads();
+        lds_barrier();
         if (rhi >= 0) {
             v4f acc0[NB], acc1[NB];
 #pragma unroll
@@ -668,7 +668,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *
     for (int c0 = 0; c0 < te.count; c0 += SM_CH) {
         // this wave's list within the chunk: (byte offset, length, a multiple of 4)
         const int2 lst = brange[((size_t)te.pad + c0 / SM_CH) * 16 + w];
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int q = 0; q < NLB; q++) {
             const int wi = tid + q * 1024;
@@ -704,7 +704,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *
         }
         // slots of the chunk past its item count are never listed
         if (c0 + SM_CH < te.count) prefetch(c0 + SM_CH);
-        __syncthreads();
+        lds_barrier();
         const unsigned char *ls = sl + lst.x + lk;
         const int len = __builtin_amdgcn_readfirstlane(lst.y);
 #pragma unroll 2
