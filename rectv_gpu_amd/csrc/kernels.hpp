@@ -586,12 +586,44 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
 constexpr int SM_CH = 128;  // items staged per chunk (8 staging threads each)
 constexpr int SM_ZW = 4;    // default slices per workgroup (template argument ZW)
 constexpr int SM_LB = 16 * (SM_CH + 4);  // list bytes per chunk (16 blocks, padded)
+
+// Weight rows of the scatter's adjoint chunks, once per geometry: for chunk ci
+// (achunk[ci] = {tile, first record, count}) and slot s < count, 16-wide rows
+// wy[r] = gauss(r - ly0 - 4 - fy), wx[c] = gauss(c - lx0 - 4 - fx) over the tile's
+// rows / cells, zero outside the window, stored as 8 float4 per slot (wy then wx).
+// The gather computes its taps with the same function, so the two operators stay
+// exact transposes of each other.
+__global__ void __launch_bounds__(1024) k_scatter_weights(const AdjItem *__restrict__ aitems,
+                                                          const int4 *__restrict__ achunk, float4 *__restrict__ wtab,
+                                                          int n, float beta) {
+    const int ci = blockIdx.x;
+    const int4 ch = achunk[ci];
+    const int sm = threadIdx.x >> 3, part = threadIdx.x & 7;
+    if (sm >= ch.z) return;
+    const int N = 2 * n, nT = N / TS;
+    const int ty0 = (ch.x / nT) * TS, tx0 = (ch.x % nT) * TS;
+    const AdjItem g = aitems[ch.y + sm];
+    int dy = (g.rc & 0xffff) - ty0;
+    if (dy < 0) dy += N;
+    int dx = (g.rc >> 16) - tx0;
+    if (dx < 0) dx += N;
+    const int l0 = part < 4 ? (dy < TS ? dy : dy - N) : (dx < TS ? dx : dx - N);
+    const float fr = part < 4 ? g.fy : g.fx;
+    float w[4];
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int a = 4 * (part & 3) + h - l0;
+        w[h] = ((unsigned)a < (unsigned)KW) ? gauss(beta, (float)(a - 4) - fr) : 0.f;
+    }
+    wtab[((size_t)ci * SM_CH + sm) * 8 + part] = make_float4(w[0], w[1], w[2], w[3]);
+}
+
 template <int NB, int ZW>   // 16-column blocks: MP <= 8*NB (NB = 1 or 2); ZW slices per workgroup
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))  // two workgroups per CU
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *__restrict__ aitems,
           const TileEntry *__restrict__ tiles, const int2 *__restrict__ crange,
           const int2 *__restrict__ brange, const unsigned char *__restrict__ blist, const float2 *__restrict__ cf,
-          int n, int nt, int MP, float beta, int ntiles, int nzb) {
+          const float4 *__restrict__ wtab, int n, int nt, int MP, int ntiles, int nzb) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
     constexpr int NC = 16 * NB;
     constexpr int HALF = 8 * NB;       // column of the imaginary part of pair 0
@@ -627,9 +659,10 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *
 #pragma unroll
         for (int b = 0; b < NB; b++) acc[zz][b] = (v4f){0.f, 0.f, 0.f, 0.f};
     // chunk staging data are prefetched into registers one chunk ahead, so the
-    // dependent global loads (item -> geometry, q, coefficients) overlap the MFMA loop
+    // dependent global loads (item -> q, coefficients) overlap the MFMA loop; the
+    // weight rows come precomputed (k_scatter_weights), one float4 per thread
     int pid = -1;
-    AdjItem pg = {0, 0, 0.f, 0.f};
+    float4 pw = make_float4(0.f, 0.f, 0.f, 0.f);
     float2 pq[ZW], pph[NPP];
     constexpr int NLB = (SM_LB + 4 * 1024 - 1) / (4 * 1024);  // list words per thread
     unsigned plw[NLB];
@@ -638,8 +671,8 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *
         const int cnt = min(SM_CH, te.count - c0);
         pid = -1;
         if (sm < cnt) {
-            pg = aitems[te.start + c0 + sm];
-            pid = pg.pid;
+            pw = wtab[((size_t)(te.pad + c0 / SM_CH) * SM_CH + sm) * 8 + part];
+            pid = aitems[te.start + c0 + sm].pid;
             const int sid = pid >> 1;
             const bool mir = pid & 1;
 #pragma unroll
@@ -675,22 +708,9 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *
             if (4 * wi < pcr.y) reinterpret_cast<unsigned *>(sl)[wi] = plw[q];
         }
         if (pid >= 0) {
-            const AdjItem g = pg;
-            int dy = (g.rc & 0xffff) - ty0;
-            if (dy < 0) dy += N;
-            int dx = (g.rc >> 16) - tx0;
-            if (dx < 0) dx += N;
-            const int ly0 = dy < TS ? dy : dy - N;
-            const int lx0 = dx < TS ? dx : dx - N;
-            // weight-table entries 2*part, 2*part+1 (zero outside the window)
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int i = 2 * part + h;
-                const int ay = i - ly0, ax = i - lx0;
-                const float dyv = (float)(ay - 4) - g.fy, dxv = (float)(ax - 4) - g.fx;
-                wyr[sm][i] = ((unsigned)ay < (unsigned)KW) ? gauss(beta, dyv) : 0.f;
-                wxr[sm][i] = ((unsigned)ax < (unsigned)KW) ? gauss(beta, dxv) : 0.f;
-            }
+            // weight rows: parts 0-3 hold wy[4 part ..], parts 4-7 wx[4 (part - 4) ..]
+            float *wd = part < 4 ? &wyr[sm][4 * part] : &wxr[sm][4 * (part - 4)];
+            *reinterpret_cast<float4 *>(wd) = pw;
 #pragma unroll
             for (int zz = 0; zz < ZW; zz++) {
 #pragma unroll

@@ -71,6 +71,7 @@ struct HostGeom {
     // chunk's lists, length): slots of the items whose window meets the block
     std::vector<int2> crange, brange;
     std::vector<unsigned char> blist;
+    std::vector<int4> achunk;  // per adjoint chunk: {tile, first record, count, 0}
     std::vector<float2> pre_f, post_a, cf;
     std::vector<float> post_f, pre_a, deap;
     float beta = 0.f;
@@ -250,6 +251,7 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         hg.crange.clear();
         hg.brange.clear();
         hg.blist.clear();
+        hg.achunk.clear();
         for (int t = 0; t < ntile; t++) {
             TileEntry &te = hg.atiles[t];
             te.pad = (int)hg.crange.size();
@@ -276,6 +278,7 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
                 const int len = (int)hg.blist.size() - base;
                 if (len > SM_LB) throw Error(-1, "internal: scatter block lists exceed their staging area");
                 hg.crange.push_back(make_int2(base, len));
+                hg.achunk.push_back(make_int4(t, te.start + c0, cn, 0));
             }
         }
     }
@@ -414,6 +417,8 @@ struct Solver {
     DBuf<TileEntry> d_fmtiles, d_atiles;
     DBuf<int2> d_crange, d_brange;
     DBuf<unsigned char> d_blist;
+    DBuf<int4> d_achunk;
+    DBuf<float4> d_wtab;  // scatter weight rows per adjoint chunk (8 float4 per slot)
     DBuf<float2> d_cf, d_pre_f, d_post_a;
     DBuf<float> d_post_f, d_pre_a, d_deap;
     int nfm_entries = 0, na_tiles = 0;
@@ -453,11 +458,13 @@ struct Solver {
         const size_t state = 3 * V * 4 + 2 * V4 * 16 + V * 4 + (size_t)nz * nt * n * 4;
         const size_t per_slice = MP * n * N * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 16 + (size_t)nt * n * 4;
         const size_t ni = 2 * (size_t)nt * n, ntile = (N / TS) * (N / TS);
-        // geo 16 B, VALU groups <= 16 B, gather slots <= 16 ints, adjoint lists <= 4 records
-        // of 16 B and block lists <= 9 bytes per item; per adjoint chunk its range, 16 block
-        // ranges and up to 3 padding bytes per block; tables per ray / pixel
+        // geo 16 B, gather slots <= 16 ints, adjoint lists <= 4 records of 16 B and block
+        // lists <= 9 bytes per item (16 B spare); per adjoint chunk its range, 16 block
+        // ranges, up to 3 padding bytes per block, its record range and SM_CH weight rows
+        // of 128 B; tables per ray / pixel
         const size_t nchk = ntile + 4 * ni / SM_CH + 1;
-        const size_t geom = ni * (16 + 16 + 64 + 64 + 9) + nchk * (8 + 16 * 8 + 16 * 3) + 3 * ntile * 16 +
+        const size_t geom = ni * (16 + 16 + 64 + 64 + 9) + nchk * (8 + 16 * 8 + 16 * 3 + 16 + SM_CH * 128) +
+                            3 * ntile * 16 +
                             (size_t)nt * 2 * MP * 8 + 8 * (size_t)n * 8 + 2 * (N + n) * 8;
         return state + per_slice * nz + geom + (1 << 20);
     }
@@ -630,6 +637,7 @@ struct Solver {
         upload(d_crange, hg.crange, bytes, st);
         upload(d_brange, hg.brange, bytes, st);
         upload(d_blist, hg.blist, bytes, st);
+        upload(d_achunk, hg.achunk, bytes, st);
         upload(d_cf, hg.cf, bytes, st);
         upload(d_pre_f, hg.pre_f, bytes, st);
         upload(d_post_f, hg.post_f, bytes, st);
@@ -637,6 +645,14 @@ struct Solver {
         upload(d_post_a, hg.post_a, bytes, st);
         upload(d_deap, hg.deap, bytes, st);
         na_tiles = (int)hg.atiles.size();
+        // scatter weight rows, computed once per geometry with the gather's window function
+        const int nchunks = (int)hg.achunk.size();
+        d_wtab.alloc((size_t)std::max(nchunks, 1) * SM_CH * 8, bytes);
+        if (nchunks > 0)
+            launch("scatter_weights", [&] {
+                hipLaunchKernelGGL(k_scatter_weights, dim3(nchunks), dim3(1024), 0, st, d_aitems.p, d_achunk.p,
+                                   d_wtab.p, n, beta);
+            });
         RTV_HIP(hipMemcpyAsync(d_g.p, g, (size_t)nz * nt * n * sizeof(float), hipMemcpyHostToDevice, st));
         RTV_HIP(hipStreamSynchronize(st));
         inputs = true;
@@ -712,11 +728,11 @@ struct Solver {
             // MP <= 16 (M <= 32): one or two 16-column blocks
             if (MP <= 8)
                 hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_aitems.p,
-                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, n, nt, MP, beta,
+                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, d_wtab.p, n, nt, MP,
                                    na_tiles, zc);
             else
                 hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_aitems.p,
-                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, n, nt, MP, beta,
+                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, d_wtab.p, n, nt, MP,
                                    na_tiles, zc);
         });
     }
@@ -1197,6 +1213,8 @@ struct Solver {
         d_crange.release(t);
         d_brange.release(t);
         d_blist.release(t);
+        d_achunk.release(t);
+        d_wtab.release(t);
         d_cf.release(t);
         d_pre_f.release(t);
         d_post_a.release(t);
