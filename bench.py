@@ -156,7 +156,8 @@ def served_bench(count, niter, titer, rate, seed=7, max_active=16, sample_frac=0
     import threading
     from rectv_gpu_amd import synthetic, batch
     specs = synthetic.serve_specs(count)
-    jobs = [synthetic.make_job(sp, niter, titer) for sp in specs]
+    # a job ends at niter or when its relative change per outer iteration drops below 1e-4
+    jobs = [synthetic.make_job(sp, niter, titer, tol=1e-4) for sp in specs]
     gaps = np.random.default_rng(seed).exponential(1.0 / rate, count)
     print("bench: serving mix of %d jobs generated" % count, file=sys.stderr, flush=True)
     srv = batch.Server(max_active=max_active, ahead=2, workers=4)
@@ -190,8 +191,8 @@ def served_bench(count, niter, titer, rate, seed=7, max_active=16, sample_frac=0
     for i in sample:
         r = batch.solve_sequential([jobs[i].fresh()])[0].result
         diff = max(diff, float(np.linalg.norm(jobs[i].result.astype(np.float64) - r) / max(np.linalg.norm(r), 1e-30)))
-    return {"workload": "%d jobs (batched mix + 5%% N=1024 Nz=32 M=16), Poisson arrivals %.0f jobs/s, niter=%d, "
-                        "titer=%d, max %d active" % (count, rate, niter, titer, max_active),
+    return {"workload": "%d jobs (batched mix + 5%% N=1024 Nz=32 M=16), Poisson arrivals %.0f jobs/s, niter=%d "
+                        "or relative change < 1e-4, titer=%d, max %d active" % (count, rate, niter, titer, max_active),
             "value": round(vox / (t1 - t0), 1), "unit": "voxel-iter/s", "wall_s": round(t1 - t0, 3),
             "latency": lat, "jobs": count, "errors": len(errors), "peak_active": srv.peak_active,
             "sample_max_rel_l2_vs_plain": diff, "sample_jobs": k}
