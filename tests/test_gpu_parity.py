@@ -44,6 +44,10 @@ def _case(name):
         n, nt, m, nz = 128, 128, 4, 8
         th = B.angles(32, 4)
         phi, center = B.realbasis(nt, m), n / 2
+    elif name == "n64_m20_two_blocks":    # 10 frame pairs: two 16-column MFMA blocks
+        n, nt, m, nz = 64, 40, 20, 2
+        th = B.angles(20, 2)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
     elif name == "n1024_column_rounds":   # 2n = 2048: two column rounds per workgroup
         n, nt, m, nz = 1024, 12, 3, 1
         th = B.angles(6, 2)
@@ -57,7 +61,8 @@ def _case(name):
     return n, nt, m, nz, th, phi, center, rng
 
 
-CASES = ["n64_rand", "n72_mixed_radix_m3", "cfg_small", "n1024_column_rounds", "n2048_column_rounds"]
+CASES = ["n64_rand", "n72_mixed_radix_m3", "cfg_small", "n64_m20_two_blocks", "n1024_column_rounds",
+         "n2048_column_rounds"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -184,6 +189,7 @@ def _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed, basis="fourier", lam
     (32, 6, 2, 2, 16, 4, 5, 2),      # three chunks: halo slabs from the previous iterate
     (64, 4, 4, 2, 32, 4, 4, 3),
     (48, 2, 1, 4, 12, 3, 3, 1),      # nzp = 1, odd m
+    (32, 4, 2, 2, 16, 19, 3, 2),     # m = 19: two MFMA column blocks, odd m
 ])
 def test_run_parity_small(gpu, n, nz, nzp, nrot, ntr, m, niter, titer):
     say("case run", n, nz, nzp, nrot * ntr, m, niter, titer)
