@@ -215,6 +215,9 @@ def main():
     ap.add_argument("--job-niter", type=int, default=16, help="outer iterations per batched / served job")
     ap.add_argument("--job-titer", type=int, default=4)
     ap.add_argument("--rate", type=float, default=20.0, help="serving arrival rate (jobs/s)")
+    ap.add_argument("--light-rate", type=float, default=2.0,
+                    help="arrival rate of the second, below-capacity serving run (jobs/s; 0 skips it)")
+    ap.add_argument("--light-jobs", type=int, default=24)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build drives exactly one GPU; --gpus %d is not supported "
@@ -228,6 +231,8 @@ def main():
             out["batched"] = batched_bench(a.jobs, a.job_niter, a.job_titer)
         else:
             out["served"] = served_bench(a.jobs, a.job_niter, a.job_titer, a.rate)
+            if a.light_rate > 0:
+                out["served_light"] = served_bench(a.light_jobs, a.job_niter, a.job_titer, a.light_rate)
         print(json.dumps(out), flush=True)
         return
 
@@ -302,6 +307,8 @@ def main():
     if a.mode == "all":
         out["batched"] = batched_bench(a.jobs, a.job_niter, a.job_titer)
         out["served"] = served_bench(a.jobs, a.job_niter, a.job_titer, a.rate)
+        if a.light_rate > 0:
+            out["served_light"] = served_bench(a.light_jobs, a.job_niter, a.job_titer, a.light_rate)
     if not a.no_cpu:
         print("bench: CPU baseline sample ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, a.titer)

@@ -96,6 +96,18 @@ def test_autotune_keeps_results(gpu):
     assert rel(b, a) < SAME_TOL
 
 
+def test_footprint_bounds_device_bytes(gpu):
+    from rectv_gpu_amd import Solver
+    for (n, nz, nzp, nrot, ntr, m) in [(64, 4, 2, 2, 32, 4), (128, 2, 2, 4, 16, 8), (96, 3, 1, 2, 20, 19)]:
+        j = _job(n, nz, nzp, nrot, ntr, m, 1, 1, seed=n + m)
+        fp = Solver.footprint(j.n, j.ntheta, j.m, j.nz, j.nzp)
+        with Solver(j.n, j.ntheta, j.m, j.nz, j.nzp, 1) as s:
+            s.set_inputs(j.data, j.theta, j.phi)
+            used = s.device_bytes()
+        say("case footprint", j.name, "bound", fp, "held", used)
+        assert 0 < used <= fp
+
+
 def test_batch_matches_alone(gpu):
     from rectv_gpu_amd.batch import solve_batch
     jobs = [_job(64, 4, 2, 2, 32, 4, 4, 2, seed=11), _job(128, 2, 2, 4, 16, 8, 3, 1, seed=12),
