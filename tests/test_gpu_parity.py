@@ -201,6 +201,19 @@ def test_run_parity_small(gpu, n, nz, nzp, nrot, ntr, m, niter, titer):
     np.testing.assert_allclose(hist[:, 3], [x["rdiff"] for x in h], rtol=1e-3)
 
 
+def test_run_in_slice_batches(gpu, monkeypatch):
+    """A scratch budget of one chunk forces the solver to run the operators over
+    three slice batches (halos, data and frame-pair offsets per batch)."""
+    n, nz, nzp, nrot, ntr, m, niter, titer = 32, 6, 2, 2, 16, 4, 3, 2
+    say("case run in slice batches", n, nz, nzp, m)
+    ref_all = _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed=77)
+    monkeypatch.setenv("RECTV_SCRATCH_MB", "0.01")
+    out, ref, hist, h = _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed=77)
+    e, e_same = rel(out, ref), rel(out, ref_all[0])
+    say("  rel L2 vs oracle", e, "vs one batch", e_same)
+    assert e < RUN_TOL and e_same < 1e-6
+
+
 def test_run_parity_small_config(gpu):
     """The plain-method small configuration: N=128, Nz=8, Nrot=4 x 32, M=4 real basis,
     20 spheres, 1% noise, lambda0=1e-3, lambda1=4, niter=32, titer=4."""
