@@ -1049,133 +1049,96 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 // P1: after the inner steps: h2 = grad fm; psi = shrink(h2 + mu/rho, lam0/rho);
 // mu += rho (h2 - psi); partial sums r = |h2 - psi|^2, s = |grad(f_old - fm)|^2
 // and ||fm - f_old||^2 (admm.cu:101-117, rectv.cu:224-237), plus ||fm||^2 (the
-// scale of the relative change used by callers that stop early).  Thread per
-// dual column (chunk, y', x') over all (n+1)^2 positions, ghost entries included.
+// scale of the relative change used by callers that stop early), over all dual
+// entries, ghost entries included.  One thread per dual entry, no state carried
+// between entries: workgroup = (chunk, z', t') plane and a band of PX_ROWS rows
+// y'; wave w takes rows y0 + w, y0 + w + 4, ..., lanes take x' = lane + 64 k.
+// Each entry reads the five extension values it needs (own, +x, +y, +t, +z) of
+// fm and of the previous iterate A straight from memory (the neighbours are
+// mostly cache hits).  Halo slabs (chunk neighbours) come from A
+// (rectv.cu:170-171).
 constexpr int PROX_W = 4;  // partial sums per workgroup
-// The M+1 dual frames of a column are split over FS threads (frame groups of
-// Q = ceil((MM+1)/FS)); a group holds the Q+1 frames iq0-1 .. iq0+Q-1 it
-// reads (clamped to [0, M-1]) for the current and the next slab.  Workgroup:
-// 64 x positions x FS groups x TV_ROWS/FS rows.
-#ifndef RTV_PROX_FS
-#define RTV_PROX_FS 1  // frame groups per dual column (build-time tuning constant)
-#endif
-template <int MM>
-__host__ __device__ constexpr int prox_fs() { return MM >= 8 ? RTV_PROX_FS : 1; }
-template <int MM>
-__global__ void __launch_bounds__(TV_THREADS)
+constexpr int PX_ROWS = 64;  // rows y' per workgroup
+__global__ void __launch_bounds__(256)
 k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__restrict__ psi,
-       float4 *__restrict__ mu, TvArgs a, double *__restrict__ partial) {
-    constexpr int FS = prox_fs<MM>();
-    constexpr int Q = (MM + 1 + FS - 1) / FS;
-    constexpr int QL = Q + 1;
-    constexpr int ROWS = TV_ROWS / FS;
-    __shared__ double red[PROX_W][TV_THREADS / 64];
+            float4 *__restrict__ mu, TvArgs a, double *__restrict__ partial) {
+    __shared__ double red[PROX_W][4];
     const int n = a.n, M = a.M, nzp = a.nzp;
-    // dual position (x', y') in [0, n]^2 of chunk c, frame group g
-    const int gx = (n + 1 + 63) / 64, gy = (n + 1 + ROWS - 1) / ROWS;
+    const int nb = (n + 1 + PX_ROWS - 1) / PX_ROWS;  // row bands per plane
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int xq = (L % gx) * 64 + (int)(threadIdx.x & 63);
-    const int g = (int)(threadIdx.x >> 6) % FS;
-    const int yq = ((L / gx) % gy) * ROWS + (int)(threadIdx.x >> 6) / FS;
-    const int c = a.chunk0 + L / (gx * gy);
-    const int iq0 = g * Q;
+    const int band = L % nb;
+    const int pl = L / nb;                  // plane: (chunk, z', t')
+    const int iq = pl % (M + 1);
+    const int zq = (pl / (M + 1)) % (nzp + 1);
+    const int c = a.chunk0 + pl / ((M + 1) * (nzp + 1));
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
     const float r = *a.rho;
     const float rinv = 1.0f / r;
     const float th = a.lam0 / r;
+    const bool flgl = c != 0, flgr = c != a.nchunk - 1;
+    const size_t plane = (size_t)n * n, S = (size_t)M * plane;
+    const size_t zb = (size_t)c * nzp;
+    // slab of extension index z' (E index) and whether it is a halo
+    auto slab = [&](int zlq, bool &halo) -> size_t {
+        int q = zlq - 1;
+        const int lo = flgl ? -1 : 0, hi = flgr ? nzp : nzp - 1;
+        q = q < lo ? lo : (q > hi ? hi : q);
+        halo = (q < 0 || q >= nzp);
+        return zb + q;
+    };
+    bool hc, hn;
+    const size_t sc0 = slab(zq, hc) * S, sn = slab(zq + 1, hn) * S;
+    const int ic = iq > 0 ? iq - 1 : 0, it = iq < M ? iq : M - 1;
+    const size_t fc = (size_t)ic * plane, ft = (size_t)it * plane;
     double sr = 0.0, ss = 0.0, sc = 0.0, sf = 0.0;
-    if (xq <= n && yq <= n) {
-        const bool flgl = c != 0, flgr = c != a.nchunk - 1;
-        const size_t plane = (size_t)n * n, S = (size_t)M * plane;
-        const size_t zb = (size_t)c * nzp;
-        auto cl = [n](int q) { return q < 0 ? 0 : (q > n - 1 ? n - 1 : q); };
-        auto clm = [M](int q) { return q < 0 ? 0 : (q > M - 1 ? M - 1 : q); };
-        const int yc = cl(yq - 1), xc = cl(xq - 1), yn = cl(yq), xn = cl(xq);
-        // slab of extension index zl' (E index), returns global slab and whether it is a halo
-        auto slab = [&](int zlq, bool &halo) -> size_t {
-            int q = zlq - 1;
-            const int lo = flgl ? -1 : 0, hi = flgr ? nzp : nzp - 1;
-            q = q < lo ? lo : (q > hi ? hi : q);
-            halo = (q < 0 || q >= nzp);
-            return zb + q;  // q == -1 -> previous chunk's last slab (zb >= 1 then)
-        };
-        // frames held: local j <-> frame clm(iq0 - 1 + j)
-        float cF[QL], cD[QL], nF[QL], nD[QL];
-        {
-            bool hl;
-            const size_t s0 = slab(0, hl);
-#pragma unroll
-            for (int j = 0; j < QL; j++) {
-                const size_t o = s0 * S + clm(iq0 - 1 + j) * plane + (size_t)yc * n + xc;
-                const float av = A[o];
-                cF[j] = hl ? av : fm[o];
-                cD[j] = hl ? 0.f : av - cF[j];
+    const int y1 = min(n + 1, (band + 1) * PX_ROWS);
+    for (int yq = band * PX_ROWS + w; yq < y1; yq += 4) {
+        const int yc = yq > 0 ? yq - 1 : 0, yn = yq < n ? yq : n - 1;
+        for (int xq = lane; xq <= n; xq += 64) {
+            const int xc = xq > 0 ? xq - 1 : 0, xn = xq < n ? xq : n - 1;
+            const size_t o0 = (size_t)yc * n + xc, ox = (size_t)yc * n + xn, oy = (size_t)yn * n + xc;
+            // extension values (F) and the change against the previous iterate (D)
+            auto FD = [&](size_t o, bool h, float &F, float &D) {
+                const float av = A[o], fv = fm[o];
+                F = h ? av : fv;
+                D = h ? 0.f : av - fv;
+            };
+            float e0, d0, xF, xD, yF, yD, tF, tD, zF, zD;
+            FD(sc0 + fc + o0, hc, e0, d0);
+            FD(sc0 + fc + ox, hc, xF, xD);
+            FD(sc0 + fc + oy, hc, yF, yD);
+            FD(sc0 + ft + o0, hc, tF, tD);
+            FD(sn + fc + o0, hn, zF, zD);
+            const float hx = a.gs * (xF - e0), hy = a.gs * (yF - e0);
+            const float ht = a.gt * ((iq < M ? tF : e0) - e0), hz = a.gs * (zF - e0);
+            const float dx = a.gs * (xD - d0), dy = a.gs * (yD - d0);
+            const float dt = a.gt * ((iq < M ? tD : d0) - d0), dz = a.gs * (zD - d0);
+            ss += (double)dx * dx + (double)dy * dy + (double)dt * dt + (double)dz * dz;
+            if (zq >= 1 && iq >= 1 && yq >= 1 && xq >= 1) {
+                sc += (double)d0 * d0;
+                sf += (double)e0 * e0;
             }
-        }
-        for (int zq = 0; zq <= nzp; zq++) {
-            bool hn, hc;
-            const size_t sn = slab(zq + 1, hn);
-            const size_t sc0 = slab(zq, hc);
-            const size_t ox = sc0 * S + (size_t)yc * n + xn;
-            const size_t oy = sc0 * S + (size_t)yn * n + xc;
-#pragma unroll
-            for (int j = 0; j < QL; j++) {
-                const size_t on = sn * S + clm(iq0 - 1 + j) * plane + (size_t)yc * n + xc;
-                const float an = A[on];
-                nF[j] = hn ? an : fm[on];
-                nD[j] = hn ? 0.f : an - nF[j];
+            const size_t d = dual_idx(a, c, zq, iq, yq, xq);
+            float4 m4 = mu[d];
+            float4 p4 = make_float4(hx + m4.x * rinv, hy + m4.y * rinv, ht + m4.z * rinv, hz + m4.w * rinv);
+            const float za = sqrtf(p4.x * p4.x + p4.y * p4.y + p4.z * p4.z + p4.w * p4.w);
+            if (za <= th) {
+                p4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                const float sh = th / za;
+                p4.x -= sh * p4.x;
+                p4.y -= sh * p4.y;
+                p4.z -= sh * p4.z;
+                p4.w -= sh * p4.w;
             }
-#pragma unroll
-            for (int jq = 0; jq < Q; jq++) {
-                const int iq = iq0 + jq;
-                if (iq <= M) {
-                    // E index iq reads frame clamp(iq-1) (local jq); iq+1 reads clamp(iq)
-                    // (local jq + 1, used for iq < M)
-                    const size_t fo = (size_t)(iq > 0 ? iq - 1 : 0) * plane;
-                    const float e0 = cF[jq];
-                    const float axv = A[ox + fo], ayv = A[oy + fo];
-                    const float xFv = hc ? axv : fm[ox + fo];
-                    const float yFv = hc ? ayv : fm[oy + fo];
-                    const float xDv = hc ? 0.f : axv - xFv, yDv = hc ? 0.f : ayv - yFv;
-                    const float hx = a.gs * (xFv - e0);
-                    const float hy = a.gs * (yFv - e0);
-                    const float ht = a.gt * ((iq < M ? cF[jq + 1] : e0) - e0);
-                    const float hz = a.gs * (nF[jq] - e0);
-                    const float d0 = cD[jq];
-                    const float dx = a.gs * (xDv - d0), dy = a.gs * (yDv - d0);
-                    const float dt = a.gt * ((iq < M ? cD[jq + 1] : d0) - d0), dz = a.gs * (nD[jq] - d0);
-                    ss += (double)dx * dx + (double)dy * dy + (double)dt * dt + (double)dz * dz;
-                    if (zq >= 1 && iq >= 1 && yq >= 1 && xq >= 1) {
-                        sc += (double)d0 * d0;
-                        sf += (double)e0 * e0;
-                    }
-                    const size_t d = dual_idx(a, c, zq, iq, yq, xq);
-                    float4 m4 = mu[d];
-                    float4 p4 = make_float4(hx + m4.x * rinv, hy + m4.y * rinv, ht + m4.z * rinv, hz + m4.w * rinv);
-                    const float za = sqrtf(p4.x * p4.x + p4.y * p4.y + p4.z * p4.z + p4.w * p4.w);
-                    if (za <= th) {
-                        p4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                    } else {
-                        const float sh = th / za;
-                        p4.x -= sh * p4.x;
-                        p4.y -= sh * p4.y;
-                        p4.z -= sh * p4.z;
-                        p4.w -= sh * p4.w;
-                    }
-                    m4.x += r * hx - r * p4.x;
-                    m4.y += r * hy - r * p4.y;
-                    m4.z += r * ht - r * p4.z;
-                    m4.w += r * hz - r * p4.w;
-                    psi[d] = p4;
-                    mu[d] = m4;
-                    const float ex = hx - p4.x, ey = hy - p4.y, et = ht - p4.z, ez = hz - p4.w;
-                    sr += (double)ex * ex + (double)ey * ey + (double)et * et + (double)ez * ez;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < QL; j++) {
-                cF[j] = nF[j];
-                cD[j] = nD[j];
-            }
+            m4.x += r * hx - r * p4.x;
+            m4.y += r * hy - r * p4.y;
+            m4.z += r * ht - r * p4.z;
+            m4.w += r * hz - r * p4.w;
+            psi[d] = p4;
+            mu[d] = m4;
+            const float ex = hx - p4.x, ey = hy - p4.y, et = ht - p4.z, ez = hz - p4.w;
+            sr += (double)ex * ex + (double)ey * ey + (double)et * et + (double)ez * ez;
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -1184,21 +1147,15 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
         sc += __shfl_down(sc, o, 64);
         sf += __shfl_down(sf, o, 64);
     }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
         red[0][w] = sr;
         red[1][w] = ss;
         red[2][w] = sc;
         red[3][w] = sf;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < PROX_W; q++) {
-            double t = 0.0;
-            for (int k = 0; k < TV_THREADS / 64; k++) t += red[q][k];
-            partial[PROX_W * (size_t)L + q] = t;
-        }
-    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < PROX_W; q++) partial[PROX_W * (size_t)L + q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
 }
 
 // The penalty rule between outer iterations (rectv.cu:220-232), on the device so

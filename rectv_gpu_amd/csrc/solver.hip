@@ -508,7 +508,7 @@ struct Solver {
         d_v.alloc((size_t)zb * MP * n * n, bytes);
         d_gq.alloc((size_t)zb * nt * n * 2, bytes);
         d_h1.alloc((size_t)zb * nt * n, bytes);
-        const size_t pb = (size_t)((n + 1 + 63) / 64) * (n + 1) * nch;  // >= prox workgroups
+        const size_t pb = (size_t)((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * nch;  // prox workgroups
         const size_t rb = (size_t)((zb * nt + T_ray - 1) / T_ray);
         d_part.alloc(PROX_W * std::max(pb, rb) + 16, bytes);
         d_red.alloc(8, bytes);
@@ -810,12 +810,10 @@ struct Solver {
         else if (M <= 16) launch_update<16>(cur, A, nxt, ta, ncb, zv0);
         else launch_update<32>(cur, A, nxt, ta, ncb, zv0);
     }
-    template <int MM>
-    void launch_prox(const float *cur, const float *A, const TvArgs &ta) {
-        const int rows = TV_ROWS / prox_fs<MM>();
-        const dim3 grid(((n + 1 + 63) / 64) * ((n + 1 + rows - 1) / rows) * nch);
+    void prox(const float *cur, const float *A, const TvArgs &ta) {
+        const dim3 grid(((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * nch);
         launch("prox", [&] {
-            hipLaunchKernelGGL(k_prox<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
+            hipLaunchKernelGGL(k_prox, grid, dim3(256), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
         });
         const int cnt = grid.x;
         launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, PROX_W); });
@@ -835,12 +833,6 @@ struct Solver {
         else if (M <= 8) launch_divchi<8>(ta);
         else if (M <= 16) launch_divchi<16>(ta);
         else launch_divchi<32>(ta);
-    }
-    void prox(const float *cur, const float *A, const TvArgs &ta) {
-        if (M <= 4) launch_prox<4>(cur, A, ta);
-        else if (M <= 8) launch_prox<8>(cur, A, ta);
-        else if (M <= 16) launch_prox<16>(cur, A, ta);
-        else launch_prox<32>(cur, A, ta);
     }
 
     // ------------------------------------------------------------ ADMM driver
