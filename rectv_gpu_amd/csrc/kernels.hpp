@@ -11,8 +11,9 @@
 //
 // Data layouts in HBM (z = slice within the current slice batch):
 //   f        [nz][M][n][n]                 f32   time frames of each slice
-//   H        [zb][MP][n][2n]               c64   row-transformed spectra (forward),
-//                                                 column-transformed (adjoint)
+//   H        [zb][MP][2n/g][n][g]          c64   row-transformed spectra (forward, g = HG),
+//                                                 column-transformed (adjoint, g = HGA
+//                                                 or 2n): groups of g adjacent columns kx
 //   G        [zb][MP][2n][2n]              c64   oversampled spectra, centred, stored
 //                                                 transposed: G[kx][ky] (column FFT rows)
 //   gq       [zb][ntheta][n][2]            c64   per sample: direct / mirror gather
@@ -39,6 +40,31 @@ constexpr int TH = TS + KW - 1; // tile + halo rows loaded by the gather
 
 __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 
+// H holds each (slice, pair) spectrum as groups of g adjacent columns kx, each
+// group [n rows y][g]; g = 2n is the plain row-major layout.  The forward
+// direction uses g = HG: the column pass then reads contiguous blocks, and the row
+// pass writes g-element row segments, which the TT rows of one workgroup complete
+// to whole cache lines.  The adjoint uses g = HGA (0: row-major), where the
+// column pass writes and the row pass reads.  Build-time tuning constants.
+#ifndef RTV_HG
+#define RTV_HG 4
+#endif
+#ifndef RTV_HGA
+#define RTV_HGA 0
+#endif
+constexpr int HG = RTV_HG, HGA = RTV_HGA;
+__host__ __device__ constexpr int h_group_adj(int N) { return HGA > 0 ? HGA : N; }
+__device__ __forceinline__ size_t h_index(size_t zj, int n, int y, int kx, int g) {
+    return ((zj * (size_t)(2 * n / g) + (size_t)(kx / g)) * (size_t)n + y) * g + (kx % g);
+}
+// element kx = jl + m * TPT of row y (row passes of a power-of-two length)
+template <int TPT, int G>
+__device__ __forceinline__ size_t h_row_off(size_t zj, int nn, int y, int jl, int m) {
+    if constexpr (TPT % G == 0) return h_index(zj, nn, y, jl, G) + (size_t)m * (TPT / G) * nn * G;
+    else if constexpr (G >= 16 * TPT) return h_index(zj, nn, y, 0, G) + jl + m * TPT;  // one group per row
+    else return h_index(zj, nn, y, jl + m * TPT, G);
+}
+
 typedef float v4f __attribute__((ext_vector_type(4)));  // MFMA accumulator fragment
 
 // Transforms per FFT workgroup for a compile-time power-of-two length L:
@@ -49,15 +75,19 @@ constexpr int FFT_WG = 512;
 __host__ __device__ constexpr int fft_rows_p2(int L) {
     return (FFT_WG / (L / FFT_E)) < 1 ? 1 : ((FFT_WG / (L / FFT_E)) < 16 ? (FFT_WG / (L / FFT_E)) : 16);
 }
-// Column passes: FFT_THREADS / (L / FFT_E) adjacent columns per workgroup (at most
-// 16), so that the strided side of the pass moves 64-byte row segments at L = 2048
-// (one workgroup per CU; LDS ~140 KB).
-#ifndef RTV_COLS_THREADS
-#define RTV_COLS_THREADS 1024  // threads per column-pass workgroup (build-time tuning constant)
+// Column passes: threads / (L / FFT_E) adjacent columns per workgroup (at most
+// 16).  Build-time tuning constants: the forward pass reads HG-column groups of H
+// (256 threads at L = 2048: 2 columns, LDS ~35 KB, four workgroups per CU); the
+// adjoint pass writes 64-byte row segments of H (1024 threads: 8 columns, one
+// workgroup per CU).
+#ifndef RTV_FCOLS_THREADS
+#define RTV_FCOLS_THREADS 256
 #endif
-__host__ __device__ constexpr int fft_cols_p2(int L) {
-    return (RTV_COLS_THREADS / (L / FFT_E)) < 1 ? 1
-           : ((RTV_COLS_THREADS / (L / FFT_E)) < 16 ? (RTV_COLS_THREADS / (L / FFT_E)) : 16);
+#ifndef RTV_ACOLS_THREADS
+#define RTV_ACOLS_THREADS 1024
+#endif
+__host__ __device__ constexpr int fft_cols_p2(int L, int threads) {
+    return (threads / (L / FFT_E)) < 1 ? 1 : ((threads / (L / FFT_E)) < 16 ? (threads / (L / FFT_E)) : 16);
 }
 
 struct SampleGeom {
@@ -87,7 +117,8 @@ struct TileEntry {
 // F1: frame pairs f_{2p} + i f_{2p+1} + deapodisation + zero-padded row FFT
 // (makecomplex/decphi, kernels_operators.cuh:86-116, is replaced by the pair
 // packing; divphi FWD + the first half of the 2D cuFFT, kernels_radonusfft.cuh:4-24,
-// radonusfft.cu:85-89).  One workgroup per (slice z, image row y, T pairs).
+// radonusfft.cu:85-89).  One workgroup per (slice z, pair p, TT image rows) for
+// power-of-two lengths, per (slice z, image row y, T pairs) otherwise.
 template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__restrict__ deap, FftPlan plan,
@@ -96,22 +127,21 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__r
     const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     if constexpr (NP > 0) {
-        // thread (t, jl): pair p0 + t; element positions jl + m*TPT of the
-        // 2n-point row; the n image pixels sit at positions 4*TPT .. 12*TPT - 1
+        // thread (t, jl): image row yt*TT + t; element positions jl + m*TPT of
+        // the 2n-point row; the n image pixels sit at positions 4*TPT .. 12*TPT - 1
         constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_rows_p2(NP);
-        const int ngr = (MP + TT - 1) / TT;
-        const int pg = L % ngr;
-        const int y = (L / ngr) % nn;
-        const int z = L / ngr / nn;
+        constexpr int nyt = nn / TT;
+        const int yt = L % nyt;
+        const int zp = L / nyt;  // z*MP + p
+        const int z = zp / MP, p = zp - z * MP;
         const int t = tid / TPT, jl = tid % TPT;
-        const int p = pg * TT + t;
-        const bool active = p < MP;
+        const int y = yt * TT + t;
         const size_t plane = (size_t)nn * nn;
         const float sg = sgn(jl);  // (-1)^position: TPT is even
         float2 v[16];
 #pragma unroll
         for (int m = 0; m < 16; m++) v[m] = make_float2(0.f, 0.f);
-        if (active) {
+        {
             const int i0 = 2 * p;
             const float *fa = f + ((size_t)z * M + i0) * plane + (size_t)y * nn + jl;
             const float sy = deap[y] * sg;
@@ -125,12 +155,9 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__r
 #pragma unroll
             for (int m = 0; m < 8; m++) v[m + 4] = make_float2(va[m] * dx[m] * sy, vb[m] * dx[m] * sy);
         }
-        fft_p2_reg<NP, false>(v, lds + t * stride, active, jl, tw + plan.tw_off);
-        if (active) {
-            float2 *Hr = H + (((size_t)z * MP + p) * nn + y) * N + jl;
+        fft_p2_reg<NP, false>(v, lds + t * stride, true, jl, tw + plan.tw_off);
 #pragma unroll
-            for (int m = 0; m < 16; m++) Hr[m * TPT] = cscale(v[m], sg);
-        }
+        for (int m = 0; m < 16; m++) H[h_row_off<TPT, HG>(zp, nn, y, jl, m)] = cscale(v[m], sg);
     } else {
         const int nthr = blockDim.x;
         const int ngr = (MP + T - 1) / T;
@@ -155,10 +182,9 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__r
         }
         __syncthreads();
         fft_lds<FFT_E, false, true>(lds, stride, Tc, plan, tw, tid, nthr);
-        float2 *Hz = H + ((size_t)z * MP + p0) * n * N + (size_t)y * N;
         for (int idx = tid; idx < Tc * N; idx += nthr) {
             const int pt = idx / N, kx = idx - pt * N;
-            Hz[(size_t)pt * n * N + kx] = cscale(lds[pt * stride + lp(kx)], sgn(kx));
+            H[h_index((size_t)z * MP + p0 + pt, n, y, kx, HG)] = cscale(lds[pt * stride + lp(kx)], sgn(kx));
         }
     }
 }
@@ -176,18 +202,18 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
     if constexpr (NP > 0) {
         // thread (t, jl), t fastest: adjacent lanes read adjacent columns; the n
         // input rows sit at positions 4*TPT .. 12*TPT - 1
-        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_cols_p2(NP);
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_cols_p2(NP, RTV_FCOLS_THREADS);
         const int ntile = N / TT;
         const int kt = L % ntile;
         const int zj = L / ntile;  // z*MP + p
         const int t = tid % TT, jl = tid / TT;
         const float sg = sgn(jl);
-        const float2 *Hs = H + (size_t)zj * nn * N + kt * TT + t + (size_t)jl * N;
+        const float2 *Hs = H + h_index(zj, nn, jl, kt * TT + t, HG);  // rows jl + m*TPT: stride HG
         float2 v[16];
 #pragma unroll
         for (int m = 0; m < 16; m++) v[m] = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int m = 0; m < 8; m++) v[m + 4] = cscale(Hs[(size_t)m * TPT * N], sg);
+        for (int m = 0; m < 8; m++) v[m + 4] = cscale(Hs[m * TPT * HG], sg);
         fft_p2_reg<NP, false>(v, lds + t * stride, true, jl, tw + plan.tw_off);
         // transposed store: the transform of column kx is row kx of G
         float2 *Gs = G + (size_t)zj * N * N + (size_t)(kt * TT + t) * N + jl;
@@ -200,10 +226,9 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
         const int kt = L % ntile;
         const int zj = L / ntile;  // z*MP + p
         const int kx0 = kt * T;
-        const float2 *Hs = H + (size_t)zj * n * N + kx0;
         for (int idx = tid; idx < T * n; idx += nthr) {
             const int t = idx % T, y = idx / T;
-            lds[t * stride + lp(y + h)] = cscale(Hs[(size_t)y * N + t], sgn(y + h));
+            lds[t * stride + lp(y + h)] = cscale(H[h_index(zj, n, y, kx0 + t, HG)], sgn(y + h));
             const int q = y < h ? y : y + n;
             lds[t * stride + lp(q)] = make_float2(0.f, 0.f);
         }
@@ -777,7 +802,7 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     if constexpr (NP > 0) {
         // thread (t, jl), t fastest; output positions 4*TPT .. 12*TPT - 1 are kept
-        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_cols_p2(NP);
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_cols_p2(NP, RTV_ACOLS_THREADS);
         const int ntile = N / TT;
         const int kt = L % ntile;
         const int zj = L / ntile;
@@ -788,9 +813,10 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
 #pragma unroll
         for (int m = 0; m < 16; m++) v[m] = cscale(Gs[m * TPT], sg);
         fft_p2_reg<NP, true>(v, lds + t * stride, true, jl, tw + plan.tw_off);
-        float2 *Hs = H + (size_t)zj * nn * N + kt * TT + t + (size_t)jl * N;
+        constexpr int GA = h_group_adj(NP);
+        float2 *Hs = H + h_index(zj, nn, jl, kt * TT + t, GA);  // rows jl + (m-4)*TPT: stride GA
 #pragma unroll
-        for (int m = 4; m < 12; m++) Hs[(size_t)(m - 4) * TPT * N] = cscale(v[m], sg);
+        for (int m = 4; m < 12; m++) Hs[(size_t)(m - 4) * TPT * GA] = cscale(v[m], sg);
     } else {
         const int nthr = blockDim.x;
         const int N = 2 * n, h = N / 4;
@@ -805,10 +831,10 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
         }
         __syncthreads();
         fft_lds<FFT_E, true, true>(lds, stride, T, plan, tw, tid, nthr);
-        float2 *Hs = H + (size_t)zj * n * N + kx0;
+        const int ga = h_group_adj(N);
         for (int idx = tid; idx < T * n; idx += nthr) {
             const int t = idx % T, y = idx / T;
-            Hs[(size_t)y * N + t] = cscale(lds[t * stride + lp(y + h)], sgn(y + h));
+            H[h_index(zj, n, y, kx0 + t, ga)] = cscale(lds[t * stride + lp(y + h)], sgn(y + h));
         }
     }
 }
@@ -830,10 +856,9 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
         const int t = tid / TPT, jl = tid % TPT;
         const int y = yt * TT + t;
         const float sg = sgn(jl);
-        const float2 *Hr = H + ((size_t)zj * nn + y) * N + jl;
         float2 u[16];
 #pragma unroll
-        for (int m = 0; m < 16; m++) u[m] = cscale(Hr[m * TPT], sg);
+        for (int m = 0; m < 16; m++) u[m] = cscale(H[h_row_off<TPT, h_group_adj(NP)>(zj, nn, y, jl, m)], sg);
         fft_p2_reg<NP, true>(u, lds + t * stride, true, jl, tw + plan.tw_off);
         const float sy = sg * deap[y];
         float2 *vr = v + ((size_t)zj * nn + y) * nn + jl;
@@ -847,10 +872,10 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
         const int zj = L / nyt;
         const int y0 = yt * T;
         const int Tc = min(T, n - y0);
-        const float2 *Hs = H + ((size_t)zj * n + y0) * N;
+        const int ga = h_group_adj(N);
         for (int idx = tid; idx < Tc * N; idx += nthr) {
             const int t = idx / N, kx = idx - t * N;
-            lds[t * stride + lp(kx)] = cscale(Hs[(size_t)t * N + kx], sgn(kx));
+            lds[t * stride + lp(kx)] = cscale(H[h_index(zj, n, y0 + t, kx, ga)], sgn(kx));
         }
         __syncthreads();
         fft_lds<FFT_E, true, true>(lds, stride, Tc, plan, tw, tid, nthr);

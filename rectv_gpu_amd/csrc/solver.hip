@@ -383,7 +383,7 @@ struct Solver {
     bool odd_n = false, odd_N = false;  // lengths with factors 3, 5, 7
     bool p2N = false;                   // 2n has a compile-time power-of-two kernel
     DBuf<float2> d_tw;
-    int T_rows = 1, T_cols = 1, T_ray = 1, T_arows = 1, st_n = 0, st_N = 0;
+    int T_rows = 1, T_cols = 1, T_fcols = 1, T_acols = 1, T_ray = 1, T_arows = 1, st_n = 0, st_N = 0;
     // state
     DBuf<float> d_f[3];
     int ia = 0;
@@ -518,9 +518,9 @@ struct Solver {
         RTV_HIP(hipStreamSynchronize(st));
         if (std::getenv("RECTV_VERBOSE"))
             fprintf(stderr,
-                    "rectv: n=%d ntheta=%d m=%d nz=%d nzp=%d lds_max=%d T_rows=%d T_cols=%d T_ray=%d T_arows=%d "
+                    "rectv: n=%d ntheta=%d m=%d nz=%d nzp=%d lds_max=%d T_rows=%d T_cols=%d/%d T_ray=%d T_arows=%d "
                     "chunks/batch=%d device bytes=%zu\n",
-                    n, nt, M, nz, nzp, lds_max, T_rows, T_cols, T_ray, T_arows, cb, bytes);
+                    n, nt, M, nz, nzp, lds_max, T_rows, T_fcols, T_acols, T_ray, T_arows, cb, bytes);
     }
 
     int fft_threads(int T, int len) const {
@@ -554,12 +554,15 @@ struct Solver {
         T_cols = 1;
         while (T_cols * 2 <= std::min(tN, 16) && N % (T_cols * 2) == 0) T_cols *= 2;
         T_arows = T_rows = std::min(tN, 16);
+        T_fcols = T_acols = T_cols;
         T_ray = std::min(tn, 16);
         // power-of-two lengths use compile-time row counts inside the kernels
         p2N = (N & (N - 1)) == 0 && N >= 64 && N <= 4096;
         if (p2N) {
             T_arows = T_rows = fft_rows_p2(N);
-            T_cols = fft_cols_p2(N);
+            T_fcols = fft_cols_p2(N, RTV_FCOLS_THREADS);
+            T_acols = fft_cols_p2(N, RTV_ACOLS_THREADS);
+            T_cols = std::max(T_fcols, T_acols);
             RTV_REQUIRE(8 * st_N * T_cols <= lds_max && 8 * st_N * T_rows <= lds_max,
                         "LDS too small for the 2n-point transforms");
         }
@@ -686,8 +689,8 @@ struct Solver {
         const size_t S = (size_t)M * n * n;
         (void)S;
         {
-            const int ngr = (MP + T_rows - 1) / T_rows;
-            const int blocks = zc * n * ngr;
+            // power-of-two lengths: (slice, pair, T_rows image rows); else (slice, row, T_rows pairs)
+            const int blocks = p2N ? zc * MP * (n / T_rows) : zc * n * ((MP + T_rows - 1) / T_rows);
             const int thr = fft_threads(T_rows, N);
             const size_t lds = (size_t)T_rows * st_N * sizeof(float2);
             launch("fwd_rows", [&] {
@@ -696,12 +699,12 @@ struct Solver {
             });
         }
         {
-            const int blocks = zc * MP * (N / T_cols);
-            const int thr = fft_threads(T_cols, N);
-            const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
+            const int blocks = zc * MP * (N / T_fcols);
+            const int thr = fft_threads(T_fcols, N);
+            const size_t lds = (size_t)T_fcols * st_N * sizeof(float2);
             launch("fwd_cols", [&] {
                 hipLaunchKernelGGL(pick_fwd_cols(N), dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p, n, MP,
-                                   T_cols, st_N);
+                                   T_fcols, st_N);
             });
         }
         gather(zc, gather_zg);
@@ -757,12 +760,12 @@ struct Solver {
     void rays_to_images(int zc) {
         scatter(zc, scatter_zw);
         {
-            const int blocks = zc * MP * (N / T_cols);
-            const int thr = fft_threads(T_cols, N);
-            const size_t lds = (size_t)T_cols * st_N * sizeof(float2);
+            const int blocks = zc * MP * (N / T_acols);
+            const int thr = fft_threads(T_acols, N);
+            const size_t lds = (size_t)T_acols * st_N * sizeof(float2);
             launch("adj_cols", [&] {
                 hipLaunchKernelGGL(pick_adj_cols(N), dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p, n, MP,
-                                   T_cols, st_N);
+                                   T_acols, st_N);
             });
         }
         {
