@@ -90,6 +90,17 @@ __host__ __device__ constexpr int fft_cols_p2(int L, int threads) {
     return (threads / (L / FFT_E)) < 1 ? 1 : ((threads / (L / FFT_E)) < 16 ? (threads / (L / FFT_E)) : 16);
 }
 
+// Cells of G inside some gather window, per row kx of G: the shortest circular
+// run of columns [lo, lo + len) (mod 2n) holding them, len = 0 for none.  The
+// forward column pass stores only these cells and the adjoint column pass reads
+// only these; every other cell of G stays zero (G is cleared when the geometry
+// is set, and the scatter writes exact zeros outside the windows).
+__device__ __forceinline__ bool in_run(int2 run, int c, int N) {
+    int d = c - run.x;
+    if (d < 0) d += N;
+    return d < run.y;
+}
+
 struct SampleGeom {
     int row0;   // first window row in the 2n grid (already mod 2n)
     int col0;   // first window column
@@ -195,7 +206,7 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__r
 template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, const float2 *__restrict__ tw,
-           int n, int M, int T, int stride) {
+           const int2 *__restrict__ runs, int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -215,10 +226,12 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
 #pragma unroll
         for (int m = 0; m < 8; m++) v[m + 4] = cscale(Hs[m * TPT * HG], sg);
         fft_p2_reg<NP, false>(v, lds + t * stride, true, jl, tw + plan.tw_off);
-        // transposed store: the transform of column kx is row kx of G
+        // transposed store: the transform of column kx is row kx of G (window cells only)
+        const int2 run = runs[kt * TT + t];
         float2 *Gs = G + (size_t)zj * N * N + (size_t)(kt * TT + t) * N + jl;
 #pragma unroll
-        for (int m = 0; m < 16; m++) Gs[m * TPT] = cscale(v[m], sg);
+        for (int m = 0; m < 16; m++)
+            if (in_run(run, jl + m * TPT, N)) Gs[m * TPT] = cscale(v[m], sg);
     } else {
         const int nthr = blockDim.x;
         const int N = 2 * n, h = N / 4;
@@ -237,7 +250,7 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
         float2 *Gs = G + (size_t)zj * N * N + (size_t)kx0 * N;
         for (int idx = tid; idx < T * N; idx += nthr) {
             const int t = idx / N, ky = idx - t * N;
-            Gs[(size_t)t * N + ky] = cscale(lds[t * stride + lp(ky)], sgn(ky));
+            if (in_run(runs[kx0 + t], ky, N)) Gs[(size_t)t * N + ky] = cscale(lds[t * stride + lp(ky)], sgn(ky));
         }
     }
 }
@@ -796,7 +809,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *
 template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, const float2 *__restrict__ tw,
-           int n, int M, int T, int stride) {
+           const int2 *__restrict__ runs, int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -809,9 +822,17 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
         const int t = tid % TT, jl = tid / TT;
         const float sg = sgn(jl);
         const float2 *Gs = G + (size_t)zj * N * N + (size_t)(kt * TT + t) * N + jl;  // row kx of G
+        // cells outside the windows are zero: their lanes all read the spectrum's
+        // first cell instead (one cache line per wave; a finite value, scaled by
+        // zero, so that the loads stay unpredicated and are issued together)
+        const int2 run = runs[kt * TT + t];
+        const float2 *G0 = G + (size_t)zj * N * N;
         float2 v[16];
 #pragma unroll
-        for (int m = 0; m < 16; m++) v[m] = cscale(Gs[m * TPT], sg);
+        for (int m = 0; m < 16; m++) {
+            const bool in = in_run(run, jl + m * TPT, N);
+            v[m] = cscale(*(in ? Gs + m * TPT : G0), in ? sg : 0.f);
+        }
         fft_p2_reg<NP, true>(v, lds + t * stride, true, jl, tw + plan.tw_off);
         constexpr int GA = h_group_adj(NP);
         float2 *Hs = H + h_index(zj, nn, jl, kt * TT + t, GA);  // rows jl + (m-4)*TPT: stride GA
@@ -827,7 +848,8 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
         const float2 *Gs = G + (size_t)zj * N * N + (size_t)kx0 * N;
         for (int idx = tid; idx < T * N; idx += nthr) {
             const int t = idx / N, ky = idx - t * N;
-            lds[t * stride + lp(ky)] = cscale(Gs[(size_t)t * N + ky], sgn(ky));
+            lds[t * stride + lp(ky)] =
+                in_run(runs[kx0 + t], ky, N) ? cscale(Gs[(size_t)t * N + ky], sgn(ky)) : make_float2(0.f, 0.f);
         }
         __syncthreads();
         fft_lds<FFT_E, true, true>(lds, stride, T, plan, tw, tid, nthr);

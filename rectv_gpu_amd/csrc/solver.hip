@@ -72,6 +72,7 @@ struct HostGeom {
     std::vector<int2> crange, brange;
     std::vector<unsigned char> blist;
     std::vector<int4> achunk;  // per adjoint chunk: {tile, first record, count, 0}
+    std::vector<int2> runs;    // per row of G: circular run {lo, len} of the columns inside some window
     std::vector<float2> pre_f, post_a, cf;
     std::vector<float> post_f, pre_a, deap;
     float beta = 0.f;
@@ -282,6 +283,65 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             }
         }
     }
+    // G cells inside some window (direct or mirrored), per row of G: the shortest
+    // circular run of columns holding them.  The column passes skip the other
+    // cells, and the scatter runs only over the tiles that meet a run.
+    {
+        std::vector<unsigned char> start((size_t)N * N, 0);  // window column starts per row
+        for (size_t id = 0; id < ni; id++) {
+            if (ftile[id] < 0) continue;
+            const SampleGeom &g = hg.geo[id];
+            for (int a = 0; a < KW; a++) start[(size_t)((g.row0 + a) % N) * N + g.col0] = 1;
+        }
+        hg.runs.assign(N, make_int2(0, 0));
+        std::vector<unsigned char> cov(N);
+        for (int r = 0; r < N; r++) {
+            const unsigned char *s = &start[(size_t)r * N];
+            int ncov = 0;
+            for (int c = 0; c < N; c++) {
+                unsigned char v = 0;
+                for (int b = 0; b < KW; b++) v |= s[(c - b + N) % N];
+                cov[c] = v;
+                ncov += v;
+            }
+            if (ncov == 0) continue;
+            if (ncov == N) {
+                hg.runs[r] = make_int2(0, N);
+                continue;
+            }
+            // the longest circular gap; scanning from a covered cell keeps gaps unbroken
+            int c0 = 0;
+            while (!cov[c0]) c0++;
+            int best = 0, bstart = 0, run = 0, rstart = 0;
+            for (int k = 1; k <= N; k++) {
+                const int c = (c0 + k) % N;
+                if (cov[c]) {
+                    run = 0;
+                    continue;
+                }
+                if (run == 0) rstart = c;
+                if (++run > best) {
+                    best = run;
+                    bstart = rstart;
+                }
+            }
+            hg.runs[r] = make_int2((bstart + best) % N, N - best);
+        }
+        auto in_run_h = [&](int r, int c) {
+            int d = c - hg.runs[r].x;
+            if (d < 0) d += N;
+            return d < hg.runs[r].y;
+        };
+        std::vector<TileEntry> act;
+        for (const TileEntry &te : hg.atiles) {
+            const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
+            bool meets = te.count > 0;
+            for (int r = ty0; r < ty0 + TS && !meets; r++)
+                for (int c = tx0; c < tx0 + TS && !meets; c++) meets = in_run_h(r, c);
+            if (meets) act.push_back(te);
+        }
+        hg.atiles.swap(act);
+    }
     // per-ray tables (takeshift kernels_radonusfft.cuh:191-198 with c = -/+(center - n/2),
     // fftshift signs, 1/n, PI/mu, 1/sqrt(n ntheta), ntheta/sqrt(M))
     const float cfl = (float)(center - n / 2.0);
@@ -415,6 +475,7 @@ struct Solver {
     DBuf<int> d_fitems;
     DBuf<AdjItem> d_aitems;
     DBuf<TileEntry> d_fmtiles, d_atiles;
+    DBuf<int2> d_runs;
     DBuf<int2> d_crange, d_brange;
     DBuf<unsigned char> d_blist;
     DBuf<int4> d_achunk;
@@ -647,7 +708,11 @@ struct Solver {
         upload(d_pre_a, hg.pre_a, bytes, st);
         upload(d_post_a, hg.post_a, bytes, st);
         upload(d_deap, hg.deap, bytes, st);
+        upload(d_runs, hg.runs, bytes, st);
         na_tiles = (int)hg.atiles.size();
+        // G is zero outside the windows of this geometry (the column passes and the
+        // scatter keep it so)
+        RTV_HIP(hipMemsetAsync(d_G.p, 0, d_G.n * sizeof(float2), st));
         // scatter weight rows, computed once per geometry with the gather's window function
         const int nchunks = (int)hg.achunk.size();
         d_wtab.alloc((size_t)std::max(nchunks, 1) * SM_CH * 8, bytes);
@@ -703,8 +768,8 @@ struct Solver {
             const int thr = fft_threads(T_fcols, N);
             const size_t lds = (size_t)T_fcols * st_N * sizeof(float2);
             launch("fwd_cols", [&] {
-                hipLaunchKernelGGL(pick_fwd_cols(N), dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p, n, MP,
-                                   T_fcols, st_N);
+                hipLaunchKernelGGL(pick_fwd_cols(N), dim3(blocks), dim3(thr), lds, st, d_H.p, d_G.p, plan_N, d_tw.p,
+                                   d_runs.p, n, MP, T_fcols, st_N);
             });
         }
         gather(zc, gather_zg);
@@ -764,8 +829,8 @@ struct Solver {
             const int thr = fft_threads(T_acols, N);
             const size_t lds = (size_t)T_acols * st_N * sizeof(float2);
             launch("adj_cols", [&] {
-                hipLaunchKernelGGL(pick_adj_cols(N), dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p, n, MP,
-                                   T_acols, st_N);
+                hipLaunchKernelGGL(pick_adj_cols(N), dim3(blocks), dim3(thr), lds, st, d_G.p, d_H.p, plan_N, d_tw.p,
+                                   d_runs.p, n, MP, T_acols, st_N);
             });
         }
         {
@@ -1205,6 +1270,7 @@ struct Solver {
         d_fitems.release(t);
         d_fmtiles.release(t);
         d_atiles.release(t);
+        d_runs.release(t);
         d_crange.release(t);
         d_brange.release(t);
         d_blist.release(t);
