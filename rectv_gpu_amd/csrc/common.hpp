@@ -77,5 +77,24 @@ __device__ __forceinline__ float2 buf_ld2(__amdgpu_buffer_rsrc_t r, unsigned vof
 __device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
+// Non-temporal stores of a kernel's output that no later access of the same kernel
+// reads.  They pay off where whole cache lines are written (partial-line streams
+// lose the merging in L2), so each call site is chosen by measurement.
+typedef float v2f_nt __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt(float2 *p, float2 v) {
+    __builtin_nontemporal_store((v2f_nt){v.x, v.y}, (v2f_nt *)p);
+}
+// Streaming (non-temporal) forms for data read or written once per kernel, so that
+// the caches keep the data that neighbouring lanes and waves read again.
+constexpr int CP_NT = 2;  // cache-policy bit NT of the buffer instructions
+__device__ __forceinline__ float buf_ld_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, CP_NT));
+}
+__device__ __forceinline__ float2 buf_ld2_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return make_float2(buf_ld_nt(r, voff, soff), buf_ld_nt(r, voff + 4, soff));
+}
+__device__ __forceinline__ void buf_st_nt(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, CP_NT);
+}
 
 }  // namespace rtv

@@ -231,7 +231,7 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
         float2 *Gs = G + (size_t)zj * N * N + (size_t)(kt * TT + t) * N + jl;
 #pragma unroll
         for (int m = 0; m < 16; m++)
-            if (in_run(run, jl + m * TPT, N)) Gs[m * TPT] = cscale(v[m], sg);
+            if (in_run(run, jl + m * TPT, N)) st_nt(Gs + m * TPT, cscale(v[m], sg));
     } else {
         const int nthr = blockDim.x;
         const int N = 2 * n, h = N / 4;
@@ -885,7 +885,7 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
         const float sy = sg * deap[y];
         float2 *vr = v + ((size_t)zj * nn + y) * nn + jl;
 #pragma unroll
-        for (int m = 4; m < 12; m++) vr[(m - 4) * TPT] = cscale(u[m], sy * deap[jl + (m - 4) * TPT]);
+        for (int m = 4; m < 12; m++) st_nt(vr + (m - 4) * TPT, cscale(u[m], sy * deap[jl + (m - 4) * TPT]));
     } else {
         const int nthr = blockDim.x;
         const int N = 2 * n, h = N / 4;
@@ -1016,7 +1016,9 @@ k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *_
 // Halo slabs (chunk neighbours) come from A, the previous outer iterate
 // (rectv.cu:170-171; clamped borders kernels_operators.cuh:4-20).  The
 // gradient values needed by the backward differences in z and t are the
-// previous slice's / frame's own values, so they are carried.
+// previous slice's / frame's own values, so they are carried.  dchi and v are
+// read, and out written, as non-temporal streams: the caches keep fm, whose
+// x / y neighbours other lanes and rows read again.
 template <int MM>
 __global__ void __launch_bounds__(TV_THREADS)
 k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__restrict__ out,
@@ -1068,7 +1070,7 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
         for (int i = 0; i < MM; i++) {
             if (i < M) {
                 // R^* part: frame i of the adjoint pairs (re: even i, im: odd i)
-                if ((i & 1) == 0 && v) vp = buf_ld2(rv, 2 * oc, (i >> 1) * 2 * pb);
+                if ((i & 1) == 0 && v) vp = buf_ld2_nt(rv, 2 * oc, (i >> 1) * 2 * pb);
                 const float rpi = (i & 1) ? vp.y : vp.x;
                 const float f0 = fz[i];
                 const unsigned so = i * pb;
@@ -1081,8 +1083,8 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
                 const float ht0 = rgt * (ftp - f0);
                 const float hz0 = rgs * (fzp[i] - f0);
                 const float dv = a.ks * (hxm - hx0) + a.ks * (hym - hy0) + a.kt * (htc - ht0) + a.ks * (hzc[i] - hz0) +
-                                 buf_ld(rd, oc, so);
-                buf_st(f0 - 0.5f * dv - 0.5f * rpi, ro, oc, so);
+                                 buf_ld_nt(rd, oc, so);
+                buf_st_nt(f0 - 0.5f * dv - 0.5f * rpi, ro, oc, so);
                 htc = ht0;
                 hzc[i] = hz0;
             }
@@ -1166,7 +1168,10 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
                 sf += (double)e0 * e0;
             }
             const size_t d = dual_idx(a, c, zq, iq, yq, xq);
-            float4 m4 = mu[d];
+            // mu, psi: streamed (non-temporal), so the caches keep the extension values of
+            // fm and A that neighbouring entries read again
+            const v4f mv = __builtin_nontemporal_load((const v4f *)(mu + d));
+            float4 m4 = make_float4(mv[0], mv[1], mv[2], mv[3]);
             float4 p4 = make_float4(hx + m4.x * rinv, hy + m4.y * rinv, ht + m4.z * rinv, hz + m4.w * rinv);
             const float za = sqrtf(p4.x * p4.x + p4.y * p4.y + p4.z * p4.z + p4.w * p4.w);
             if (za <= th) {
@@ -1182,8 +1187,8 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
             m4.y += r * hy - r * p4.y;
             m4.z += r * ht - r * p4.z;
             m4.w += r * hz - r * p4.w;
-            psi[d] = p4;
-            mu[d] = m4;
+            __builtin_nontemporal_store((v4f){p4.x, p4.y, p4.z, p4.w}, (v4f *)(psi + d));
+            __builtin_nontemporal_store((v4f){m4.x, m4.y, m4.z, m4.w}, (v4f *)(mu + d));
             const float ex = hx - p4.x, ey = hy - p4.y, et = ht - p4.z, ez = hz - p4.w;
             sr += (double)ex * ex + (double)ey * ey + (double)et * et + (double)ez * ez;
         }
