@@ -109,8 +109,9 @@ int rtv_history(rtv_solver *h, long long first, int count, double *rows);
 /* on != 0: each outer iteration is captured once as a HIP graph and replayed. */
 int rtv_set_graph(rtv_solver *h, int on);
 /* Times the launch-shape candidates of the gather and scatter on this solver's
- * geometry and keeps the fastest (cached per shape in the process).  Results are
- * bitwise independent of the choice.  Outputs (may be NULL): slices per workgroup. */
+ * geometry and keeps the fastest (cached in the process per n, M and power-of-two
+ * class of ntheta).  Results are bitwise independent of the choice.  Outputs (may
+ * be NULL): slices per workgroup. */
 int rtv_autotune(rtv_solver *h, int *gather_slices, int *scatter_slices);
 /* Runs niter[j] outer iterations of every solver hs[j] (distinct, inputs set),
  * interleaved round robin on their own streams; returns when all have finished. */

@@ -10,7 +10,9 @@ change ||f(k+1) - f(k)|| / ||f(k+1)|| between outer iterations.
 pool of host threads (the host builds a job's geometry tables while earlier
 jobs already iterate), every job's outer iteration is captured once as a HIP
 graph, and the iterations of all jobs are enqueued a few at a time on their own
-streams so that their kernels overlap on the device.  ``batch_iterate`` (the C
+streams so that their kernels overlap on the device.  Launch shapes are tuned
+(``Solver.autotune``) for jobs with n >= AUTOTUNE_MIN_N; the tuned shapes are
+cached per size class, so a batch tunes once per class.  ``batch_iterate`` (the C
 entry point ``rtv_batch_iterate``) does the same round robin for solvers a
 caller has already prepared.
 
@@ -95,13 +97,24 @@ def _default_factory(job):
     return Solver(job.n, job.ntheta, job.m, job.nz, job.nzp, 1)
 
 
-def prepare(job, factory=_default_factory, graph=True, autotune=True):
-    """Create the job's solver, upload inputs and geometry, zero the state."""
+AUTOTUNE_MIN_N = 512  # autotune="auto": tune jobs of at least this n (small jobs keep the defaults)
+
+
+def _wants_tuning(job, autotune):
+    if autotune == "auto":
+        return job.n >= AUTOTUNE_MIN_N
+    return bool(autotune)
+
+
+def prepare(job, factory=_default_factory, graph=True, autotune="auto"):
+    """Create the job's solver, upload inputs and geometry, zero the state.
+    autotune: True, False or "auto" (only jobs with n >= AUTOTUNE_MIN_N, where the
+    tuning launches are small against the job's own work)."""
     s = factory(job)
     try:
         s.set_inputs(job.data, job.theta, job.phi, job.rot_center)
         s.reset()
-        if autotune:
+        if _wants_tuning(job, autotune):
             s.autotune()
             s.reset()
         if graph:
@@ -121,7 +134,7 @@ def finish(job, s):
     s.free()
 
 
-def solve_batch(jobs, workers=8, graph=True, autotune=True, budget=None, ahead=4):
+def solve_batch(jobs, workers=8, graph=True, autotune="auto", budget=None, ahead=4):
     """Solve uneven jobs together (see the module docstring): all jobs are
     submitted at once to a Server with a slot for every job, so host
     preparation (geometry tables, uploads) of later jobs overlaps the device
@@ -182,7 +195,7 @@ class Server:
     ``footprint`` gives the device bytes a job is charged (Job.footprint).
     """
 
-    def __init__(self, budget=None, max_active=32, ahead=2, workers=4, graph=True, autotune=True,
+    def __init__(self, budget=None, max_active=32, ahead=2, workers=4, graph=True, autotune="auto",
                  factory=_default_factory, footprint=None):
         self.budget = budget
         self.max_active = max_active

@@ -1134,12 +1134,18 @@ struct Solver {
     // geometry (HIP events, best of 3 after a warm-up launch).  Every candidate
     // does the same arithmetic in the same order, so results do not depend on
     // the choice.  Choices are cached per shape for the life of the process.
+    // Tuned shapes are shared per (n, M, projection-count class): the choice moves
+    // little with the slice count and the exact projection count, and a batch of
+    // uneven jobs then tunes once per class rather than once per job.
     struct TuneKey {
-        int n, nt, M, zb;
-        bool operator<(const TuneKey &o) const {
-            return std::tie(n, nt, M, zb) < std::tie(o.n, o.nt, o.M, o.zb);
-        }
+        int n, ntc, M;
+        bool operator<(const TuneKey &o) const { return std::tie(n, ntc, M) < std::tie(o.n, o.ntc, o.M); }
     };
+    static int nt_class(int nt) {
+        int c = 0;
+        while ((1 << (c + 1)) <= nt) c++;
+        return c;  // floor(log2 nt)
+    }
     static std::map<TuneKey, std::pair<int, int>> &tune_cache() {
         static std::map<TuneKey, std::pair<int, int>> c;
         return c;
@@ -1153,7 +1159,7 @@ struct Solver {
         hipEvent_t a = ev(), b = ev();
         f();  // warm-up
         float best = 1e30f;
-        for (int r = 0; r < 3; r++) {
+        for (int r = 0; r < 2; r++) {
             RTV_HIP(hipEventRecord(a, st));
             f();
             RTV_HIP(hipEventRecord(b, st));
@@ -1168,7 +1174,7 @@ struct Solver {
     }
     void autotune(int *zg_out, int *zw_out) {
         RTV_REQUIRE(inputs, "inputs not set (call set_inputs first)");
-        const TuneKey key{n, nt, M, zb};
+        const TuneKey key{n, nt_class(nt), M};
         std::pair<int, int> choice{-1, -1};
         {
             std::lock_guard<std::mutex> lk(tune_mutex());
@@ -1183,8 +1189,8 @@ struct Solver {
             forward_to_rays(d_f[ia].p, zc, d_g.p, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);
             float bestg = 1e30f, bests = 1e30f;
             int zg = GM_ZG, zw = SM_ZW;
-            for (int c : {1, 2, 4, 8, 16, 32}) {
-                if (c > 1 && c / 2 >= zc) break;
+            for (int c : {4, 8, 16, 32}) {
+                if (c > 4 && c / 2 >= zc) break;
                 const float t = time_best([&] { gather(zc, c); });
                 if (t < bestg) {
                     bestg = t;

@@ -153,3 +153,25 @@ def test_failed_job_does_not_stop_others_and_oversized_is_rejected():
     assert bad.error and "bad inputs" in bad.error
     assert big.error and "budget" in big.error
     assert srv.used == 0 and FakeSolver.live == 0
+
+
+def test_autotune_policy():
+    """autotune="auto" tunes only jobs with n >= AUTOTUNE_MIN_N; True / False force it."""
+    from rectv_gpu_amd import batch
+
+    class Tracking(FakeSolver):
+        tuned = []
+
+        def autotune(self):
+            Tracking.tuned.append(self.job.n)
+            return 8, 4
+
+    small = Job(np.zeros((2, 8, 64), np.float32), np.zeros(8, np.float32), np.zeros((2, 16), np.float32), niter=1)
+    big = Job(np.zeros((1, 4, batch.AUTOTUNE_MIN_N), np.float32), np.zeros(4, np.float32),
+              np.zeros((2, 8), np.float32), niter=1)
+    for job, mode, want in [(small, "auto", []), (big, "auto", [big.n]), (small, True, [small.n]), (big, False, [])]:
+        Tracking.tuned = []
+        s = batch.prepare(job, factory=lambda j: Tracking(j), graph=False, autotune=mode)
+        s.free()
+        assert Tracking.tuned == want, (job.n, mode)
+
