@@ -292,6 +292,7 @@ constexpr int GM_ITEMS = 256;  // items per workgroup: 16 waves x 16
 constexpr int GM_THREADS = 1024;
 constexpr int GM_KS = 4;       // max steps of 4 cells per block (the host packs blocks so)
 constexpr int GM_ZG = 8;       // default slices per workgroup (runtime argument zg)
+constexpr int GM_WYP = 16;     // padded row-tap entries per item (3 zeros, 9 taps, 4 zeros)
 template <int NB>              // 16-column blocks: MP <= 8*NB
 __global__ void __launch_bounds__(GM_THREADS)
 k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
@@ -304,7 +305,9 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     // 4 spare cells: the last 4-cell step of row TH-1 may read up to 3 cells past
     // the region (their weights are zero; the spare cells are zeroed)
     __shared__ float tile[(CELLS + 4) * NCP];
-    __shared__ float wty[GM_ITEMS][KW];  // the 9 window taps along rows / cells
+    // the 9 window taps along rows, stored at [3 + a] with zeros at 0-2 and 12-15, so
+    // that the row loop reads them at fixed offsets without a bounds check; along cells
+    __shared__ float wty[GM_ITEMS][GM_WYP];
     __shared__ float wtx[GM_ITEMS][KW];
     __shared__ int sid[GM_ITEMS];
     __shared__ int sly[GM_ITEMS], slx[GM_ITEMS];
@@ -317,9 +320,12 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     const TileEntry te = tiles[e];
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
     const int cnt = te.count;
-    // per-thread tile offsets within one slice's spectra (all pairs of a slice
-    // span MP*4n^2 < 2^31 elements)
-    int tofs[NLD];
+    // per-thread tile byte offsets within one slice's spectra (all pairs of a slice
+    // span MP*4n^2*8 < 2^32 bytes); entries outside the region or of absent pairs read
+    // the slice's first element and are not stored (the absent pairs' columns of the
+    // tile are zeroed once), so the loads need no predication
+    unsigned tofs[NLD];
+    unsigned keep = 0;  // bit q: entry q is stored
 #pragma unroll
     for (int q = 0; q < NLD; q++) {
         const int idx = tid + q * GM_THREADS;
@@ -329,25 +335,35 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
         if (gy >= N) gy -= N;
         int gx = tx0 + c;
         if (gx >= N) gx -= N;
-        tofs[q] = (idx < HALF * CELLS && p < MP) ? (p * N + gy) * N + gx : -1;
+        const bool ok = idx < HALF * CELLS && p < MP;
+        tofs[q] = ok ? 8u * (unsigned)((p * N + gy) * N + gx) : 0u;
+        keep |= ok ? (1u << q) : 0u;
     }
     float2 stv[NLD];
     auto load_tile = [&](int z) {
-        const float2 *Gz = G + (size_t)z * MP * N * N;
+        const char *Gz = reinterpret_cast<const char *>(G + (size_t)z * MP * N * N);
 #pragma unroll
-        for (int q = 0; q < NLD; q++) stv[q] = tofs[q] >= 0 ? Gz[tofs[q]] : make_float2(0.f, 0.f);
+        for (int q = 0; q < NLD; q++) stv[q] = *reinterpret_cast<const float2 *>(Gz + tofs[q]);
     };
     auto store_tile = [&] {
 #pragma unroll
         for (int q = 0; q < NLD; q++) {
-            const int idx = tid + q * GM_THREADS;
-            if (idx < HALF * CELLS) {
+            if (keep & (1u << q)) {
+                const int idx = tid + q * GM_THREADS;
                 const int p = idx / CELLS, rc = idx - p * CELLS;
                 tile[rc * NCP + p] = stv[q].x;
                 tile[rc * NCP + HALF + p] = stv[q].y;
             }
         }
     };
+    // columns of absent pairs (MP < HALF) stay zero
+    for (int i = tid; i < CELLS * HALF; i += GM_THREADS) {
+        const int rc = i / HALF, p = i - rc * HALF;
+        if (p >= MP) {
+            tile[rc * NCP + p] = 0.f;
+            tile[rc * NCP + HALF + p] = 0.f;
+        }
+    }
     load_tile(z0);
     // stage the items: 4 threads per item (rows taps 0-4 / 5-8, cell taps 0-4 / 5-8)
     {
@@ -368,9 +384,13 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
             slx[it] = lx0;
         }
         const float fr = (part < 2) ? g.fy : g.fx;
-        float *wt = (part < 2) ? &wty[it][0] : &wtx[it][0];
+        float *wt = (part < 2) ? &wty[it][3] : &wtx[it][0];
         const int a0 = (part & 1) ? 5 : 0, a1 = (part & 1) ? KW : 5;
         for (int a = a0; a < a1; a++) wt[a] = gauss(beta, (float)(a - 4) - fr);
+        if (part == 0)
+            for (int a = 0; a < 3; a++) wty[it][a] = 0.f;
+        if (part == 1)
+            for (int a = 3 + KW; a < GM_WYP; a++) wty[it][a] = 0.f;
         if (tid < 4 * NCP) tile[CELLS * NCP + tid] = 0.f;
     }
     lds_barrier();
@@ -413,6 +433,10 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
         wxv[ks] = (ks < nks && (unsigned)ax < (unsigned)KW && rhi >= 0) ? wtx[b0 + li][ax] : 0.f;
     }
     const int myly = sly[b0 + li];
+    // row taps of this lane's item from the block's first row rlo (empty slots: any
+    // in-range row; their cell taps are zero) and the tile row of rlo
+    const float *wrow = &wty[b0 + li][3 + max(rlo - myly, -3)];
+    const float *tb = tile + (rlo * TH + clo + lk) * NCP + li;
     float2 *out = gq + (size_t)z0 * nt * n * 2;
     for (int zz = 0; zz < zc; zz++) {
         if (zz > 0) lds_barrier();  // previous slice's tile fully read
@@ -424,10 +448,11 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
 #pragma unroll
             for (int b = 0; b < NB; b++) acc0[b] = acc1[b] = (v4f){0.f, 0.f, 0.f, 0.f};
             // a 9-cell window spans exactly 3 steps of 4 cells; blocks span 3 or 4
-            for (int r = rlo; r <= rhi; r++) {
-                const int ay = r - myly;
-                const float wy = (unsigned)ay < (unsigned)KW ? wty[b0 + li][ay] : 0.f;
-                const float *trow = tile + (r * TH + clo + lk) * NCP + li;
+#pragma unroll
+            for (int rr = 0; rr < 12; rr++) {  // blocks span at most 12 rows
+                if (rr > rhi - rlo) break;
+                const float wy = wrow[rr];
+                const float *trow = tb + rr * TH * NCP;
                 float bv[GM_KS][NB];
 #pragma unroll
                 for (int ks = 0; ks < 3; ks++)
