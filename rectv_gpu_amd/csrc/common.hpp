@@ -58,6 +58,15 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// v + (v of lane ^ 16), within each 32-lane half (ds_swizzle, no address VGPRs)
+__device__ __forceinline__ float add_xor16(float v) {
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x401F));
+}
+// v + (v of lane ^ 32); bp = 4 * (lane ^ 32), computed once by the caller
+__device__ __forceinline__ float add_xor32(float v, int bp) {
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(bp, __builtin_bit_cast(int, v)));
+}
+
 // Raw buffer access: a wave-uniform base in a buffer descriptor (SGPRs), a
 // 32-bit per-lane byte offset and a wave-uniform byte offset.  Keeps per-lane
 // address state at one VGPR where a loop walks many uniform sub-arrays.

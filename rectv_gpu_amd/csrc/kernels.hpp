@@ -433,6 +433,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
         wxv[ks] = (ks < nks && (unsigned)ax < (unsigned)KW && rhi >= 0) ? wtx[b0 + li][ax] : 0.f;
     }
     const int myly = sly[b0 + li];
+    const int bp32 = 4 * (l ^ 32);  // ds_bpermute address of the other half's lane
     // row taps of this lane's item from the block's first row rlo (empty slots: any
     // in-range row; their cell taps are zero) and the tile row of rlo
     const float *wrow = &wty[b0 + li][3 + max(rlo - myly, -3)];
@@ -494,10 +495,8 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
                         sy += cfv[b][v].x * d;
                     }
                 }
-            sx += __shfl_xor(sx, 16, 64);
-            sy += __shfl_xor(sy, 16, 64);
-            sx += __shfl_xor(sx, 32, 64);
-            sy += __shfl_xor(sy, 32, 64);
+            sx = add_xor32(add_xor16(sx), bp32);
+            sy = add_xor32(add_xor16(sy), bp32);
             if (lk == 0 && oid >= 0) out[(size_t)zz * nt * n * 2 + oid] = make_float2(sx, sy);
         }
     }
