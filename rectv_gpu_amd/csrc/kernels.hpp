@@ -28,6 +28,7 @@
 // becomes index arithmetic mod 2n.
 #pragma once
 #include "common.hpp"
+#include <type_traits>
 #include "fft_lds.hpp"
 
 namespace rtv {
@@ -449,35 +450,44 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
 #pragma unroll
             for (int b = 0; b < NB; b++) acc0[b] = acc1[b] = (v4f){0.f, 0.f, 0.f, 0.f};
             // a 9-cell window spans exactly 3 steps of 4 cells; blocks span 3 or 4
+            // one fully unrolled row loop per block height (9 to 12 rows), so that the
+            // LDS operands sit at immediate offsets from two base addresses
+            auto rows = [&](auto nr) {
 #pragma unroll
-            for (int rr = 0; rr < 12; rr++) {  // blocks span at most 12 rows
-                if (rr > rhi - rlo) break;
-                const float wy = wrow[rr];
-                const float *trow = tb + rr * TH * NCP;
-                float bv[GM_KS][NB];
+                for (int rr = 0; rr < decltype(nr)::value; rr++) {
+                    const float wy = wrow[rr];
+                    const float *trow = tb + rr * TH * NCP;
+                    float bv[GM_KS][NB];
 #pragma unroll
-                for (int ks = 0; ks < 3; ks++)
+                    for (int ks = 0; ks < 3; ks++)
 #pragma unroll
-                    for (int b = 0; b < NB; b++) bv[ks][b] = trow[4 * ks * NCP + 16 * b];
-                if (nks > 3) {
+                        for (int b = 0; b < NB; b++) bv[ks][b] = trow[4 * ks * NCP + 16 * b];
+                    if (nks > 3) {
 #pragma unroll
-                    for (int b = 0; b < NB; b++) bv[3][b] = trow[12 * NCP + 16 * b];
-                }
+                        for (int b = 0; b < NB; b++) bv[3][b] = trow[12 * NCP + 16 * b];
+                    }
 #pragma unroll
-                for (int ks = 0; ks < 3; ks++) {
-                    const float av = wy * wxv[ks];
+                    for (int ks = 0; ks < 3; ks++) {
+                        const float av = wy * wxv[ks];
 #pragma unroll
-                    for (int b = 0; b < NB; b++) {
-                        if (ks & 1) acc1[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[ks][b], av, acc1[b], 0, 0, 0);
-                        else acc0[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[ks][b], av, acc0[b], 0, 0, 0);
+                        for (int b = 0; b < NB; b++) {
+                            if (ks & 1) acc1[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[ks][b], av, acc1[b], 0, 0, 0);
+                            else acc0[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[ks][b], av, acc0[b], 0, 0, 0);
+                        }
+                    }
+                    if (nks > 3) {
+                        const float av = wy * wxv[3];
+#pragma unroll
+                        for (int b = 0; b < NB; b++)
+                            acc1[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[3][b], av, acc1[b], 0, 0, 0);
                     }
                 }
-                if (nks > 3) {
-                    const float av = wy * wxv[3];
-#pragma unroll
-                    for (int b = 0; b < NB; b++)
-                        acc1[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[3][b], av, acc1[b], 0, 0, 0);
-                }
+            };
+            switch (rhi - rlo) {
+                case 8: rows(std::integral_constant<int, 9>{}); break;
+                case 9: rows(std::integral_constant<int, 10>{}); break;
+                case 10: rows(std::integral_constant<int, 11>{}); break;
+                default: rows(std::integral_constant<int, 12>{}); break;
             }
             // lane (li, lk): D[16*b + 4*lk + v][li]; column c is pair c % HALF, imaginary
             // part for c >= HALF
