@@ -692,7 +692,7 @@ __global__ void __launch_bounds__(1024) k_scatter_weights(const AdjItem *__restr
 
 template <int NB, int ZW>   // 16-column blocks: MP <= 8*NB (NB = 1 or 2); ZW slices per workgroup
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))  // two workgroups per CU
-k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *__restrict__ aitems,
+k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const int2 *__restrict__ arec,
           const TileEntry *__restrict__ tiles, const int2 *__restrict__ crange,
           const int2 *__restrict__ brange, const unsigned char *__restrict__ blist, const float2 *__restrict__ cf,
           const float4 *__restrict__ wtab, int n, int nt, int MP, int ntiles, int nzb) {
@@ -733,31 +733,35 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *
     // chunk staging data are prefetched into registers one chunk ahead, so the
     // dependent global loads (item -> q, coefficients) overlap the MFMA loop; the
     // weight rows come precomputed (k_scatter_weights), one float4 per thread
-    int pid = -1;
+    bool has = false;  // this thread's slot holds a record of the chunk
     float4 pw = make_float4(0.f, 0.f, 0.f, 0.f);
+    const size_t sq = (size_t)nt * n * 2;  // float2 per slice of gq
     float2 pq[ZW], pph[NPP];
     constexpr int NLB = (SM_LB + 4 * 1024 - 1) / (4 * 1024);  // list words per thread
     unsigned plw[NLB];
     int2 pcr = make_int2(0, 0);
     auto prefetch = [&](int c0) {
         const int cnt = min(SM_CH, te.count - c0);
-        pid = -1;
-        if (sm < cnt) {
-            pw = wtab[((size_t)(te.pad + c0 / SM_CH) * SM_CH + sm) * 8 + part];
-            pid = aitems[te.start + c0 + sm].pid;
-            const int sid = pid >> 1;
-            const bool mir = pid & 1;
+        has = sm < cnt;
+        if (has) {
+            // slot sm's weight rows (8 float4 per slot: thread tid holds part tid & 7)
+            pw = wtab[(size_t)(te.pad + c0 / SM_CH) * SM_CH * 8 + tid];
+            // record: q's float2 index within a slice | mirror flag, coefficient row
+            const int2 rec = arec[te.start + c0 + sm];
+            const unsigned qb = 8u * (unsigned)(rec.x & ~1);  // byte offset of q in a slice
+            const bool mir = rec.x & 1;
 #pragma unroll
             for (int zz = 0; zz < ZW; zz++) {
-                float2 q = zz < zc ? gq[((size_t)(z0 + zz) * nt * n + sid) * 2] : make_float2(0.f, 0.f);
+                float2 q = zz < zc ? *reinterpret_cast<const float2 *>(
+                                         reinterpret_cast<const char *>(gq + (size_t)(z0 + zz) * sq) + qb)
+                                   : make_float2(0.f, 0.f);
                 if (mir) q.y = -q.y;
                 pq[zz] = q;
             }
-            const float2 *ph = cf + ((size_t)(sid / n) * 2 + (pid & 1)) * MP;
 #pragma unroll
             for (int q = 0; q < NPP; q++) {
                 const int j = part + 8 * q;
-                pph[q] = j < MP ? ph[j] : make_float2(0.f, 0.f);
+                pph[q] = j < MP ? cf[rec.y + j] : make_float2(0.f, 0.f);
             }
         }
         // the chunk's block lists: words [pcr.x/4, (pcr.x + pcr.y)/4) of blist (4-byte aligned)
@@ -779,7 +783,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const AdjItem *
             const int wi = tid + q * 1024;
             if (4 * wi < pcr.y) reinterpret_cast<unsigned *>(sl)[wi] = plw[q];
         }
-        if (pid >= 0) {
+        if (has) {
             // weight rows: parts 0-3 hold wy[4 part ..], parts 4-7 wx[4 (part - 4) ..]
             float *wd = part < 4 ? &wyr[sm][4 * part] : &wxr[sm][4 * (part - 4)];
             *reinterpret_cast<float4 *>(wd) = pw;

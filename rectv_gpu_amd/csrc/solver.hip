@@ -65,6 +65,7 @@ struct HostGeom {
     std::vector<int> fitems;  // forward items per origin tile for the matrix-core gather (-1 = padding)
     std::vector<int> asids;
     std::vector<AdjItem> aitems;  // asids with their window geometry
+    std::vector<int2> arec;       // per adjoint record: {2 * sample | mirror, first coefficient of its row}
     std::vector<TileEntry> fmtiles, atiles;  // atiles[].pad: first chunk of the tile
     // adjoint chunks (SM_CH items of a tile): crange = (byte offset, byte length) of the
     // chunk's block lists in blist; brange per (chunk, 4x4 block) = (offset within the
@@ -240,9 +241,13 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         }
         for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
         hg.aitems.resize(hg.asids.size());
+        hg.arec.resize(hg.asids.size());
+        const int MPh = (M + 1) / 2;
         for (size_t a = 0; a < hg.asids.size(); a++) {
-            const SampleGeom &g = hg.geo[hg.asids[a]];
-            hg.aitems[a] = AdjItem{hg.asids[a], g.row0 | (g.col0 << 16), g.fy, g.fx};
+            const int pid = hg.asids[a], sid = pid >> 1, mir = pid & 1;
+            const SampleGeom &g = hg.geo[pid];
+            hg.aitems[a] = AdjItem{pid, g.row0 | (g.col0 << 16), g.fy, g.fx};
+            hg.arec[a] = make_int2((2 * sid) | mir, ((sid / n) * 2 + mir) * MPh);
         }
         hg.atiles.resize(ntile);
         for (int t = 0; t < ntile; t++) hg.atiles[t] = TileEntry{t, off[t], cnt[t], 0};
@@ -474,6 +479,7 @@ struct Solver {
     DBuf<SampleGeom> d_geo;
     DBuf<int> d_fitems;
     DBuf<AdjItem> d_aitems;
+    DBuf<int2> d_arec;
     DBuf<TileEntry> d_fmtiles, d_atiles;
     DBuf<int2> d_runs;
     DBuf<int2> d_crange, d_brange;
@@ -519,14 +525,14 @@ struct Solver {
         const size_t state = 3 * V * 4 + 2 * V4 * 16 + V * 4 + (size_t)nz * nt * n * 4;
         const size_t per_slice = MP * n * N * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 16 + (size_t)nt * n * 4;
         const size_t ni = 2 * (size_t)nt * n, ntile = (N / TS) * (N / TS);
-        // geo 16 B, gather slots <= 16 ints, adjoint lists <= 4 records of 16 B and block
-        // lists <= 9 bytes per item (16 B spare); per adjoint chunk its range, 16 block
-        // ranges, up to 3 padding bytes per block, its record range and SM_CH weight rows
-        // of 128 B; tables per ray / pixel
+        // geo 16 B, gather slots <= 16 ints, adjoint lists <= 4 records of 16 + 8 B and
+        // block lists <= 9 bytes per item (16 B spare); per adjoint chunk its range, 16
+        // block ranges, up to 3 padding bytes per block, its record range and SM_CH weight
+        // rows of 128 B; tables per ray / pixel and per row of the oversampled grid
         const size_t nchk = ntile + 4 * ni / SM_CH + 1;
-        const size_t geom = ni * (16 + 16 + 64 + 64 + 9) + nchk * (8 + 16 * 8 + 16 * 3 + 16 + SM_CH * 128) +
+        const size_t geom = ni * (16 + 16 + 64 + 32 + 64 + 9) + nchk * (8 + 16 * 8 + 16 * 3 + 16 + SM_CH * 128) +
                             3 * ntile * 16 +
-                            (size_t)nt * 2 * MP * 8 + 8 * (size_t)n * 8 + 2 * (N + n) * 8;
+                            (size_t)nt * 2 * MP * 8 + 8 * (size_t)n * 8 + 2 * (N + n) * 8 + N * 8;
         return state + per_slice * nz + geom + (1 << 20);
     }
 
@@ -697,6 +703,7 @@ struct Solver {
             fprintf(stderr, "rectv: gather items %zu in %zu slots, %d entries\n", real, hg.fitems.size(), nfm_entries);
         }
         upload(d_aitems, hg.aitems, bytes, st);
+        upload(d_arec, hg.arec, bytes, st);
         upload(d_atiles, hg.atiles, bytes, st);
         upload(d_crange, hg.crange, bytes, st);
         upload(d_brange, hg.brange, bytes, st);
@@ -795,11 +802,11 @@ struct Solver {
         launch("scatter", [&] {
             // MP <= 16 (M <= 32): one or two 16-column blocks
             if (MP <= 8)
-                hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_aitems.p,
+                hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_arec.p,
                                    d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, d_wtab.p, n, nt, MP,
                                    na_tiles, zc);
             else
-                hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_aitems.p,
+                hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_arec.p,
                                    d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, d_wtab.p, n, nt, MP,
                                    na_tiles, zc);
         });
@@ -1273,6 +1280,7 @@ struct Solver {
         d_red.release(t);
         d_geo.release(t);
         d_aitems.release(t);
+        d_arec.release(t);
         d_fitems.release(t);
         d_fmtiles.release(t);
         d_atiles.release(t);
