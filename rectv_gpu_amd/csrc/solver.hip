@@ -1196,7 +1196,7 @@ struct Solver {
             forward_to_rays(d_f[ia].p, zc, d_g.p, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);
             float bestg = 1e30f, bests = 1e30f;
             int zg = GM_ZG, zw = SM_ZW;
-            for (int c : {4, 8, 16, 32}) {
+            for (int c : {4, 8, 16, 32, 64}) {
                 if (c > 4 && c / 2 >= zc) break;
                 const float t = time_best([&] { gather(zc, c); });
                 if (t < bestg) {

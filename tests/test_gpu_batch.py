@@ -89,7 +89,7 @@ def test_autotune_keeps_results(gpu):
         s.reset()
         zg, zw = s.autotune()
         say("  choice gather slices", zg, "scatter slices", zw)
-        assert zg in (1, 2, 4, 8, 16, 32) and zw in (2, 4, 8)
+        assert zg in (1, 2, 4, 8, 16, 32, 64) and zw in (2, 4, 8)
         s.reset()
         s.iterate(j.niter, j.titer, j.lambda0 * j.lambda1, j.lambda1)
         b = s.result()
