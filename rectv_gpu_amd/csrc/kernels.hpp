@@ -1142,10 +1142,11 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 // y'; wave w takes rows y0 + w, y0 + w + 4, ..., lanes take x' = lane + 64 k.
 // Each entry reads the five extension values it needs (own, +x, +y, +t, +z) of
 // fm and of the previous iterate A straight from memory (the neighbours are
-// mostly cache hits).  Halo slabs (chunk neighbours) come from A
+// mostly cache hits: blocks run plane after plane over one band of rows, so the +t
+// and +z planes are read again by the next blocks).  Halo slabs come from A
 // (rectv.cu:170-171).
 constexpr int PROX_W = 4;  // partial sums per workgroup
-constexpr int PX_ROWS = 64;  // rows y' per workgroup
+constexpr int PX_ROWS = 16;  // rows y' per workgroup
 __global__ void __launch_bounds__(256)
 k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__restrict__ psi,
             float4 *__restrict__ mu, TvArgs a, double *__restrict__ partial) {
@@ -1153,8 +1154,11 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
     const int n = a.n, M = a.M, nzp = a.nzp;
     const int nb = (n + 1 + PX_ROWS - 1) / PX_ROWS;  // row bands per plane
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int band = L % nb;
-    const int pl = L / nb;                  // plane: (chunk, z', t')
+    // planes fastest, row bands slowest: the +t and +z neighbour planes of a block's
+    // rows are read by the blocks that start right after it, while they are in cache
+    const int npl = (M + 1) * (nzp + 1) * (int)(gridDim.x / (nb * (M + 1) * (nzp + 1)));
+    const int band = L / npl;
+    const int pl = L % npl;                 // plane: (chunk, z', t')
     const int iq = pl % (M + 1);
     const int zq = (pl / (M + 1)) % (nzp + 1);
     const int c = a.chunk0 + pl / ((M + 1) * (nzp + 1));
