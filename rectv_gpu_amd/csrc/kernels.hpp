@@ -45,13 +45,14 @@ __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 // group [n rows y][g]; g = 2n is the plain row-major layout.  The forward
 // direction uses g = HG: the column pass then reads contiguous blocks, and the row
 // pass writes g-element row segments, which the TT rows of one workgroup complete
-// to whole cache lines.  The adjoint uses g = HGA (0: row-major), where the
-// column pass writes and the row pass reads.  Build-time tuning constants.
+// to whole cache lines.  The adjoint uses g = HGA (0: row-major): its column pass
+// writes the groups, and its row pass reads each group's block of rows whole
+// through LDS.  Build-time tuning constants.
 #ifndef RTV_HG
 #define RTV_HG 4
 #endif
 #ifndef RTV_HGA
-#define RTV_HGA 0
+#define RTV_HGA 4
 #endif
 constexpr int HG = RTV_HG, HGA = RTV_HGA;
 __host__ __device__ constexpr int h_group_adj(int N) { return HGA > 0 ? HGA : N; }
@@ -79,13 +80,12 @@ __host__ __device__ constexpr int fft_rows_p2(int L) {
 // Column passes: threads / (L / FFT_E) adjacent columns per workgroup (at most
 // 16).  Build-time tuning constants: the forward pass reads HG-column groups of H
 // (256 threads at L = 2048: 2 columns, LDS ~35 KB, four workgroups per CU); the
-// adjoint pass writes 64-byte row segments of H (1024 threads: 8 columns, one
-// workgroup per CU).
+// adjoint pass writes HGA-column groups of H (256 threads as well).
 #ifndef RTV_FCOLS_THREADS
 #define RTV_FCOLS_THREADS 256
 #endif
 #ifndef RTV_ACOLS_THREADS
-#define RTV_ACOLS_THREADS 1024
+#define RTV_ACOLS_THREADS 256
 #endif
 __host__ __device__ constexpr int fft_cols_p2(int L, int threads) {
     return (threads / (L / FFT_E)) < 1 ? 1 : ((threads / (L / FFT_E)) < 16 ? (threads / (L / FFT_E)) : 16);
@@ -916,9 +916,35 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
         const int t = tid / TPT, jl = tid % TPT;
         const int y = yt * TT + t;
         const float sg = sgn(jl);
+        constexpr int GA = h_group_adj(NP);
         float2 u[16];
+        if constexpr (GA < N) {
+            // grouped layout: each group of GA columns holds the workgroup's TT rows as one
+            // contiguous block, so the rows are read block by block (16 bytes per lane,
+            // consecutive lanes on consecutive bytes) into the LDS row buffers, from
+            // which each thread takes its 16 elements
+            static_assert(GA % 2 == 0, "16-byte reads of whole column pairs");
+            constexpr int NTH = TT * TPT, GROUP = TT * GA;
+            const int y0 = yt * TT;
+            const float2 *Hz = H + (size_t)zj * N * nn;
 #pragma unroll
-        for (int m = 0; m < 16; m++) u[m] = cscale(H[h_row_off<TPT, h_group_adj(NP)>(zj, nn, y, jl, m)], sg);
+            for (int k = 0; k < 8; k++) {
+                const int f = 2 * (tid + k * NTH);  // float2 index in the workgroup's rows, group-major
+                const int g = f / GROUP, rem = f - g * GROUP;
+                const int r = rem / GA, c = rem - r * GA;
+                const float4 q = *reinterpret_cast<const float4 *>(Hz + ((size_t)g * nn + y0 + r) * GA + c);
+                const int kx = g * GA + c;
+                lds[r * stride + lp(kx)] = make_float2(q.x, q.y);
+                lds[r * stride + lp(kx + 1)] = make_float2(q.z, q.w);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < 16; m++) u[m] = cscale(lds[t * stride + lp(jl + m * TPT)], sg);
+            __syncthreads();
+        } else {
+#pragma unroll
+            for (int m = 0; m < 16; m++) u[m] = cscale(H[h_row_off<TPT, GA>(zj, nn, y, jl, m)], sg);
+        }
         fft_p2_reg<NP, true>(u, lds + t * stride, true, jl, tw + plan.tw_off);
         const float sy = sg * deap[y];
         float2 *vr = v + ((size_t)zj * nn + y) * nn + jl;
