@@ -1165,7 +1165,7 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 // scale of the relative change used by callers that stop early), over all dual
 // entries, ghost entries included.  One thread per dual entry, no state carried
 // between entries: workgroup = (chunk, z', t') plane and a band of PX_ROWS rows
-// y'; wave w takes rows y0 + w, y0 + w + 4, ..., lanes take x' = lane + 64 k.
+// y'; wave w takes rows y0 + w, y0 + w + PX_WAVES, ..., lanes take x' = lane + 64 k.
 // Each entry reads the five extension values it needs (own, +x, +y, +t, +z) of
 // fm and of the previous iterate A straight from memory (the neighbours are
 // mostly cache hits: blocks run plane after plane over one band of rows, so the +t
@@ -1173,10 +1173,14 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 // (rectv.cu:170-171).
 constexpr int PROX_W = 4;  // partial sums per workgroup
 constexpr int PX_ROWS = 16;  // rows y' per workgroup
-__global__ void __launch_bounds__(256)
+#ifndef RTV_PX_WAVES
+#define RTV_PX_WAVES 16
+#endif
+constexpr int PX_WAVES = RTV_PX_WAVES;  // waves per workgroup
+__global__ void __launch_bounds__(64 * PX_WAVES)
 k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__restrict__ psi,
             float4 *__restrict__ mu, TvArgs a, double *__restrict__ partial) {
-    __shared__ double red[PROX_W][4];
+    __shared__ double red[PROX_W][PX_WAVES];
     const int n = a.n, M = a.M, nzp = a.nzp;
     const int nb = (n + 1 + PX_ROWS - 1) / PX_ROWS;  // row bands per plane
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -1209,7 +1213,7 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
     const size_t fc = (size_t)ic * plane, ft = (size_t)it * plane;
     double sr = 0.0, ss = 0.0, sc = 0.0, sf = 0.0;
     const int y1 = min(n + 1, (band + 1) * PX_ROWS);
-    for (int yq = band * PX_ROWS + w; yq < y1; yq += 4) {
+    for (int yq = band * PX_ROWS + w; yq < y1; yq += PX_WAVES) {
         const int yc = yq > 0 ? yq - 1 : 0, yn = yq < n ? yq : n - 1;
         for (int xq = lane; xq <= n; xq += 64) {
             const int xc = xq > 0 ? xq - 1 : 0, xn = xq < n ? xq : n - 1;
@@ -1275,7 +1279,11 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
     }
     __syncthreads();
     if (threadIdx.x == 0)
-        for (int q = 0; q < PROX_W; q++) partial[PROX_W * (size_t)L + q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
+        for (int q = 0; q < PROX_W; q++) {
+            double acc = 0.0;
+            for (int k = 0; k < PX_WAVES; k++) acc += red[q][k];
+            partial[PROX_W * (size_t)L + q] = acc;
+        }
 }
 
 // The penalty rule between outer iterations (rectv.cu:220-232), on the device so

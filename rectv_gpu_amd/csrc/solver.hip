@@ -888,7 +888,7 @@ struct Solver {
     void prox(const float *cur, const float *A, const TvArgs &ta) {
         const dim3 grid(((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * nch);
         launch("prox", [&] {
-            hipLaunchKernelGGL(k_prox, grid, dim3(256), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
+            hipLaunchKernelGGL(k_prox, grid, dim3(64 * PX_WAVES), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
         });
         const int cnt = grid.x;
         launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, PROX_W); });
