@@ -74,8 +74,15 @@ typedef float v4f __attribute__((ext_vector_type(4)));  // MFMA accumulator frag
 // = 512 threads keeps LDS near 70 KB so that two workgroups share a CU and one
 // transforms while the other loads or stores.
 constexpr int FFT_WG = 512;
-__host__ __device__ constexpr int fft_rows_p2(int L) {
-    return (FFT_WG / (L / FFT_E)) < 1 ? 1 : ((FFT_WG / (L / FFT_E)) < 16 ? (FFT_WG / (L / FFT_E)) : 16);
+// Per-ray transforms: 256-thread workgroups (4 rays at n = 1024) measured 7%
+// faster than 512 threads (128 threads: the same as 256); the 2n-point row
+// passes are fastest at 512.
+#ifndef RTV_RAY_WG
+#define RTV_RAY_WG 256
+#endif
+constexpr int RAY_WG = RTV_RAY_WG;  // threads of the per-ray transform workgroups
+__host__ __device__ constexpr int fft_rows_p2(int L, int wg = FFT_WG) {
+    return (wg / (L / FFT_E)) < 1 ? 1 : ((wg / (L / FFT_E)) < 16 ? (wg / (L / FFT_E)) : 16);
 }
 // Column passes: threads / (L / FFT_E) adjacent columns per workgroup (at most
 // 16).  Build-time tuning constants: the forward pass reads HG-column groups of H
@@ -535,7 +542,7 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
     if constexpr (NP > 0) {
         // thread (t, jl) of ray r0 + t holds detector positions s = jl + m*TPT,
         // which are both the outputs of one transform and the inputs of the next
-        constexpr int nn = NP, TPT = NP / 16, TT = fft_rows_p2(NP);
+        constexpr int nn = NP, TPT = NP / 16, TT = fft_rows_p2(NP, RAY_WG);
         const int t = tid / TPT, jl = tid % TPT;
         const int ray = blockIdx.x * TT + t;
         const bool active = ray < nrays;

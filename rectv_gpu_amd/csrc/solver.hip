@@ -634,7 +634,7 @@ struct Solver {
                         "LDS too small for the 2n-point transforms");
         }
         if ((n & (n - 1)) == 0 && n >= 64 && n <= 4096) {
-            T_ray = fft_rows_p2(n);
+            T_ray = fft_rows_p2(n, RAY_WG);
             RTV_REQUIRE(8 * st_n * T_ray <= lds_max, "LDS too small for the n-point transforms");
         }
         const int big = lds_max;
