@@ -17,7 +17,8 @@
 //   G        [zb][MP][2n][2n]              c64   oversampled spectra, centred, stored
 //                                                 transposed: G[kx][ky] (column FFT rows)
 //   gq       [zb][ntheta][n][2]            c64   per sample: direct / mirror gather
-//                                                 sums (forward); slot 0 = q (adjoint)
+//                                                 sums (forward)
+//   q        [zb][ntheta][n]               c64   per sample: the scatter's input (adjoint)
 //   v        [zb][MP][n][n]                c64   adjoint frame pairs (re = f_{2p}, im = f_{2p+1})
 //   psi, mu  [nchunk][nzp+1][M+1][n+1][n+1] f32x4 dual variables (x, y, t, z)
 // The oversampled grid index kb in [0, 2n) carries the centred frequency
@@ -526,11 +527,11 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
 //        kernels_operators.cuh:135-149), optional residual norm partials.
 //   ADJ: q[s] = post_a[s] * FFT(pre_a[s'] * h1[s'])  (radonusfft.cu:115-121 and
 //        the 1/n, s == 0 rule of the adjoint gather, kernels_radonusfft.cuh:102-108),
-//        written to slot 0 of sample s.
+//        written to q[s] (8 bytes per sample: whole cache lines).
 enum { RAY_FWD = 1, RAY_SUBG = 2, RAY_ADJ = 4 };
 template <int NP>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restrict__ h1in,
+k_ray(const float2 *__restrict__ gq, float2 *__restrict__ q, const float *__restrict__ g, const float *__restrict__ h1in,
       float *__restrict__ h1out, const float2 *__restrict__ pre_f, const float *__restrict__ post_f,
       const float *__restrict__ pre_a, const float2 *__restrict__ post_a, FftPlan plan,
       const float2 *__restrict__ tw, int n, int nrays, int T, int stride, int mode,
@@ -584,7 +585,7 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
             if (active) {
 #pragma unroll
                 for (int m = 0; m < 16; m++)
-                    gq[(o0 + m * TPT) * 2] = (m > 0 || jl > 0) ? cmul(v[m], post_a[jl + m * TPT]) : make_float2(0.f, 0.f);
+                    q[o0 + m * TPT] = (m > 0 || jl > 0) ? cmul(v[m], post_a[jl + m * TPT]) : make_float2(0.f, 0.f);
             }
         }
     } else {
@@ -624,7 +625,7 @@ k_ray(float2 *__restrict__ gq, const float *__restrict__ g, const float *__restr
             const int t = idx / n, s = idx - t * n;
             float2 v = make_float2(0.f, 0.f);
             if (s != 0) v = cmul(lds[t * stride + lp(s)], post_a[s]);
-            gq[((size_t)(r0 + t) * n + s) * 2] = v;
+            q[(size_t)(r0 + t) * n + s] = v;
         }
     }
     }
@@ -699,7 +700,7 @@ __global__ void __launch_bounds__(1024) k_scatter_weights(const AdjItem *__restr
 
 template <int NB, int ZW>   // 16-column blocks: MP <= 8*NB (NB = 1 or 2); ZW slices per workgroup
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))  // two workgroups per CU
-k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const int2 *__restrict__ arec,
+k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__restrict__ arec,
           const TileEntry *__restrict__ tiles, const int2 *__restrict__ crange,
           const int2 *__restrict__ brange, const unsigned char *__restrict__ blist, const float2 *__restrict__ cf,
           const float4 *__restrict__ wtab, int n, int nt, int MP, int ntiles, int nzb) {
@@ -742,7 +743,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const int2 *__r
     // weight rows come precomputed (k_scatter_weights), one float4 per thread
     bool has = false;  // this thread's slot holds a record of the chunk
     float4 pw = make_float4(0.f, 0.f, 0.f, 0.f);
-    const size_t sq = (size_t)nt * n * 2;  // float2 per slice of gq
+    const size_t sq = (size_t)nt * n;  // float2 per slice of q
     float2 pq[ZW], pph[NPP];
     constexpr int NLB = (SM_LB + 4 * 1024 - 1) / (4 * 1024);  // list words per thread
     unsigned plw[NLB];
@@ -753,14 +754,14 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ gq, const int2 *__r
         if (has) {
             // slot sm's weight rows (8 float4 per slot: thread tid holds part tid & 7)
             pw = wtab[(size_t)(te.pad + c0 / SM_CH) * SM_CH * 8 + tid];
-            // record: q's float2 index within a slice | mirror flag, coefficient row
+            // record: item 2 * sample | mirror flag, coefficient row
             const int2 rec = arec[te.start + c0 + sm];
-            const unsigned qb = 8u * (unsigned)(rec.x & ~1);  // byte offset of q in a slice
+            const unsigned qb = 4u * (unsigned)(rec.x & ~1);  // byte offset of the sample's q in a slice
             const bool mir = rec.x & 1;
 #pragma unroll
             for (int zz = 0; zz < ZW; zz++) {
                 float2 q = zz < zc ? *reinterpret_cast<const float2 *>(
-                                         reinterpret_cast<const char *>(gq + (size_t)(z0 + zz) * sq) + qb)
+                                         reinterpret_cast<const char *>(qs + (size_t)(z0 + zz) * sq) + qb)
                                    : make_float2(0.f, 0.f);
                 if (mir) q.y = -q.y;
                 pq[zz] = q;

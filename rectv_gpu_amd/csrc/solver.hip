@@ -471,7 +471,7 @@ struct Solver {
     int gather_zg = GM_ZG, scatter_zw = SM_ZW;
     // scratch (one batch of cb chunks = zb slices)
     int cb = 1, zb = 1;
-    DBuf<float2> d_H, d_G, d_v, d_gq;
+    DBuf<float2> d_H, d_G, d_v, d_gq, d_q;
     DBuf<float> d_h1;
     DBuf<double> d_part, d_red;
     // geometry
@@ -523,7 +523,7 @@ struct Solver {
         const size_t V = (size_t)nz * M * n * n;
         const size_t V4 = nch * (nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1);
         const size_t state = 3 * V * 4 + 2 * V4 * 16 + V * 4 + (size_t)nz * nt * n * 4;
-        const size_t per_slice = MP * n * N * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 16 + (size_t)nt * n * 4;
+        const size_t per_slice = MP * n * N * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 24 + (size_t)nt * n * 4;
         const size_t ni = 2 * (size_t)nt * n, ntile = (N / TS) * (N / TS);
         // geo 16 B, gather slots <= 16 ints, adjoint lists <= 4 records of 16 + 8 B and
         // block lists <= 9 bytes per item (16 B spare); per adjoint chunk its range, 16
@@ -563,7 +563,7 @@ struct Solver {
         d_g.alloc((size_t)nz * nt * n, bytes);
         // scratch plan
         const size_t per_slice = (size_t)MP * n * N * 8 + (size_t)MP * N * N * 8 + (size_t)MP * n * n * 8 +
-                                 (size_t)nt * n * 16 + (size_t)nt * n * 4;
+                                 (size_t)nt * n * 24 + (size_t)nt * n * 4;
         size_t freeb = 0, totb = 0;
         RTV_HIP(hipMemGetInfo(&freeb, &totb));
         double budget = 0.45 * (double)freeb;
@@ -574,6 +574,7 @@ struct Solver {
         d_G.alloc((size_t)zb * MP * N * N, bytes);
         d_v.alloc((size_t)zb * MP * n * n, bytes);
         d_gq.alloc((size_t)zb * nt * n * 2, bytes);
+        d_q.alloc((size_t)zb * nt * n, bytes);
         d_h1.alloc((size_t)zb * nt * n, bytes);
         const size_t pb = (size_t)((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * nch;  // prox workgroups
         const size_t rb = (size_t)((zb * nt + T_ray - 1) / T_ray);
@@ -756,7 +757,7 @@ struct Solver {
     }
 
     // ------------------------------------------------------------ operator pipeline
-    // forward: f slices [z0, z0+zc) -> polar samples in d_gq (and h1 / residual / q per mode)
+    // forward: f slices [z0, z0+zc) -> polar samples in d_gq (and h1 / residual / q in d_q per mode)
     void forward_to_rays(const float *f, int zc, const float *gdat, float *h1out, int mode, double *partial) {
         const size_t S = (size_t)M * n * n;
         (void)S;
@@ -802,11 +803,11 @@ struct Solver {
         launch("scatter", [&] {
             // MP <= 16 (M <= 32): one or two 16-column blocks
             if (MP <= 8)
-                hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_arec.p,
+                hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
                                    d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, d_wtab.p, n, nt, MP,
                                    na_tiles, zc);
             else
-                hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_gq.p, d_arec.p,
+                hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
                                    d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, d_wtab.p, n, nt, MP,
                                    na_tiles, zc);
         });
@@ -823,12 +824,12 @@ struct Solver {
         const int thr = fft_threads(T_ray, n);
         const size_t lds = (size_t)T_ray * st_n * sizeof(float2);
         launch("ray", [&] {
-            hipLaunchKernelGGL(pick_ray(n), dim3(blocks), dim3(thr), lds, st, d_gq.p, gdat, h1in, h1out, d_pre_f.p,
+            hipLaunchKernelGGL(pick_ray(n), dim3(blocks), dim3(thr), lds, st, d_gq.p, d_q.p, gdat, h1in, h1out, d_pre_f.p,
                                d_post_f.p, d_pre_a.p, d_post_a.p, plan_n, d_tw.p, n, nrays, T_ray, st_n, mode, partial);
         });
     }
 
-    // adjoint: d_gq (q samples of zc slices) -> d_v images
+    // adjoint: d_q (q samples of zc slices) -> d_v images
     void rays_to_images(int zc) {
         scatter(zc, scatter_zw);
         {
@@ -1204,7 +1205,7 @@ struct Solver {
                     zg = c;
                 }
             }
-            ray(zc, d_g.p, nullptr, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);  // q back in gq
+            ray(zc, d_g.p, nullptr, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);  // q for the scatter timings
             for (int c : {2, 4, 8}) {
                 const float t = time_best([&] { scatter(zc, c); });
                 if (t < bests) {
@@ -1275,6 +1276,7 @@ struct Solver {
         d_G.release(t);
         d_v.release(t);
         d_gq.release(t);
+        d_q.release(t);
         d_h1.release(t);
         d_part.release(t);
         d_red.release(t);
