@@ -220,8 +220,8 @@ def main():
     ap.add_argument("--light-jobs", type=int, default=24)
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build drives exactly one GPU; --gpus %d is not supported "
-              "(the z-chunk sharding across GPUs is described in DESIGN.md, not built)" % a.gpus, file=sys.stderr)
+        print("bench.py: the solver drives exactly one GPU; --gpus %d is not supported "
+              "(sharding the z-chunks across GPUs is not implemented)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     os.environ.setdefault("RECTV_QUIET", "1")
     from rectv_gpu_amd import Solver, synthetic

@@ -36,7 +36,7 @@ int rtv_device_count(void);
 
 /* Replaces rectv::rectv(n, ntheta, m, nz, nzp, ngpus), src/cuda/rectv.cu:5-59
  * (SWIG ctor src/cuda/rectv.i:21).  ngpus is clamped like the reference
- * (min(ngpus, nz/nzp), rectv.cu:9) and then to the one GPU this build drives.
+ * (min(ngpus, nz/nzp), rectv.cu:9) and then to the one GPU this library drives.
  * Fails (instead of truncating, rectv.cu:162) when nz % nzp != 0. */
 int rtv_create(int n, int ntheta, int m, int nz, int nzp, int ngpus, rtv_solver **out);
 

@@ -1,6 +1,6 @@
 """CPU restatement of rectv_gpu's 4D reconstruction path (NumPy, FP64 by default).
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP product path:
+TEST INFRASTRUCTURE ONLY.  The HIP product path is checked against this module:
 only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import it.  The product (``rectv_gpu_amd``) never imports it and
 has no CPU fallback.
@@ -359,7 +359,7 @@ def div(fn, h2, tau, lambda1):
 
 
 # ---------------------------------------------------------------------------
-# ADMM (admm.cu) and the driver (rectv.cu)
+# ADMM (admm.cu) and its outer loop (rectv.cu)
 # ---------------------------------------------------------------------------
 
 def solve_reg(h2, mu, lam, rho):

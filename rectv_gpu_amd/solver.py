@@ -34,7 +34,7 @@ class Solver(rectv):
 
     Parameters are those of the reference: n (detector width / slice size),
     ntheta (total number of projections), m (number of temporal basis
-    functions), nz (slices), nzp (slices per chunk), ngpus (accepted; this build
+    functions), nz (slices), nzp (slices per chunk), ngpus (accepted; this library
     drives one GPU).
     """
 
