@@ -82,6 +82,12 @@ constexpr int FFT_WG = 512;
 #define RTV_RAY_WG 256
 #endif
 constexpr int RAY_WG = RTV_RAY_WG;  // threads of the per-ray transform workgroups
+// Launch bound of the FFT pass kernels by their length template NP: compile-time
+// powers of two keep FFT_THREADS; the runtime-planned path (NP = 0) runs at most
+// FFT_WG threads, which lets it hold a stage's butterflies in registers without
+// spilling (up to 256 VGPRs); NP = -1 is the same path for transforms that need
+// more than FFT_WG threads (length > 16 FFT_WG).
+__host__ __device__ constexpr int fft_bound(int NP) { return NP == 0 ? FFT_WG : FFT_THREADS; }
 __host__ __device__ constexpr int fft_rows_p2(int L, int wg = FFT_WG) {
     return (wg / (L / FFT_E)) < 1 ? 1 : ((wg / (L / FFT_E)) < 16 ? (wg / (L / FFT_E)) : 16);
 }
@@ -140,7 +146,7 @@ struct TileEntry {
 // radonusfft.cu:85-89).  One workgroup per (slice z, pair p, TT image rows) for
 // power-of-two lengths, per (slice z, image row y, T pairs) otherwise.
 template <int NP>
-__global__ void __launch_bounds__(FFT_THREADS)
+__global__ void __launch_bounds__(fft_bound(NP))
 k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__restrict__ deap, FftPlan plan,
            const float2 *__restrict__ tw, int n, int M, int MP, int T, int stride) {
     extern __shared__ float2 lds[];
@@ -213,7 +219,7 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__r
 // cuFFT of radonusfft.cu:89 with the post-checkerboard).  One workgroup per
 // (z, j, tile of T adjacent columns).
 template <int NP>
-__global__ void __launch_bounds__(FFT_THREADS)
+__global__ void __launch_bounds__(fft_bound(NP))
 k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, const float2 *__restrict__ tw,
            const int2 *__restrict__ runs, int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
@@ -530,7 +536,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
 //        written to q[s] (8 bytes per sample: whole cache lines).
 enum { RAY_FWD = 1, RAY_SUBG = 2, RAY_ADJ = 4 };
 template <int NP>
-__global__ void __launch_bounds__(FFT_THREADS)
+__global__ void __launch_bounds__(fft_bound(NP))
 k_ray(const float2 *__restrict__ gq, float2 *__restrict__ q, const float *__restrict__ g, const float *__restrict__ h1in,
       float *__restrict__ h1out, const float2 *__restrict__ pre_f, const float *__restrict__ post_f,
       const float *__restrict__ pre_a, const float2 *__restrict__ post_a, FftPlan plan,
@@ -853,7 +859,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
 // A2: column inverse FFT of G keeping the n centre rows (cuFFT INVERSE of
 // radonusfft.cu:128 with the checkerboards, first half).
 template <int NP>
-__global__ void __launch_bounds__(FFT_THREADS)
+__global__ void __launch_bounds__(fft_bound(NP))
 k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, const float2 *__restrict__ tw,
            const int2 *__restrict__ runs, int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];
@@ -910,7 +916,7 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
 // A3: row inverse FFT keeping the n centre columns, deapodisation (divphi ADJ,
 // kernels_radonusfft.cuh:19-23).  One workgroup per (z, j, T rows).
 template <int NP>
-__global__ void __launch_bounds__(FFT_THREADS)
+__global__ void __launch_bounds__(fft_bound(NP))
 k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__restrict__ deap, FftPlan plan,
            const float2 *__restrict__ tw, int n, int M, int T, int stride) {
     extern __shared__ float2 lds[];

@@ -24,7 +24,8 @@ static const double PI_EXACT = 3.14159265358979323846;
 
 // ---------------------------------------------------------------- kernel pickers
 // FFT kernels are instantiated for power-of-two lengths (compile-time index
-// math) and once with a runtime plan (NP = 0) for every other length.
+// math) and with a runtime plan for every other length: NP = 0 (at most FFT_WG
+// threads per workgroup) or NP = -1 (longer transforms, up to FFT_THREADS).
 #define RTV_PICKER(NAME, K)                                                                        \
     static decltype(&K<0>) NAME(int len) {                                                         \
         switch (len) {                                                                             \
@@ -35,7 +36,7 @@ static const double PI_EXACT = 3.14159265358979323846;
             case 1024: return K<1024>;                                                             \
             case 2048: return K<2048>;                                                             \
             case 4096: return K<4096>;                                                             \
-            default: return K<0>;                                                                  \
+            default: return fft_tpt(len, FFT_E) > FFT_WG ? K<-1> : K<0>;                          \
         }                                                                                          \
     }
 RTV_PICKER(pick_fwd_rows, k_fwd_rows)
