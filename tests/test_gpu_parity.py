@@ -40,6 +40,10 @@ def _case(name):
         th = B.angles(15, 2)
         phi = rng.standard_normal((m, 2 * nt)).astype(np.float32) / np.sqrt(nt)
         center = n / 2 - 0.5
+    elif name == "n384_mixed_radix":      # 2n = 768 = 2^8 * 3: runtime-planned transforms, 8-column workgroups
+        n, nt, m, nz = 384, 24, 4, 1
+        th = B.angles(12, 2)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
     elif name == "cfg_small":
         n, nt, m, nz = 128, 128, 4, 8
         th = B.angles(32, 4)
@@ -61,8 +65,8 @@ def _case(name):
     return n, nt, m, nz, th, phi, center, rng
 
 
-CASES = ["n64_rand", "n72_mixed_radix_m3", "cfg_small", "n64_m20_two_blocks", "n1024_column_rounds",
-         "n2048_column_rounds"]
+CASES = ["n64_rand", "n72_mixed_radix_m3", "n384_mixed_radix", "cfg_small", "n64_m20_two_blocks",
+         "n1024_column_rounds", "n2048_column_rounds"]
 
 
 @pytest.mark.parametrize("name", CASES)
