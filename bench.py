@@ -45,11 +45,14 @@ def kernel_bytes(name, n, nt, m, zc, nzp):
     }.get(name)
 
 
-def byte_model(n, nt, m, nz, nzp, titer):
-    """Compulsory HBM bytes of one outer iteration (fused-pipeline model)."""
+def byte_model(n, nt, m, nz, nzp, titer, pairs=False):
+    """Compulsory HBM bytes of one outer iteration (fused-pipeline model).  With
+    pairs=False the model counts the reference algorithm's M*Nz spectra; with
+    pairs=True the ceil(M/2)*Nz frame-pair spectra this solver transforms (the
+    same arithmetic: two real frames per complex transform)."""
     V = n * n * m * nz
     V4 = (n + 1) ** 2 * (m + 1) * (nzp + 1) * nz // nzp
-    S = m * nz
+    S = ((m + 1) // 2 if pairs else m) * nz
     G = (2 * n) ** 2 * 8
     H = 2 * n * n * 8
     Gs = n * nt * nz
@@ -57,20 +60,40 @@ def byte_model(n, nt, m, nz, nzp, titer):
     return titer * inner + 12 * V + 48 * V4
 
 
-def copy_probe(nbytes=1 << 30):
-    """A device-to-device copy of known size (for calibrating rocprofv3's
-    FETCH_SIZE / WRITE_SIZE counters against a plain copy kernel)."""
+def _hip():
     import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
+    return ctypes, ctypes.CDLL("libamdhip64.so")
+
+
+def copy_probe(nbytes=1 << 30, reps=5):
+    """A device-to-device copy of known size on the null stream (for calibrating
+    rocprofv3's FETCH_SIZE / WRITE_SIZE counters against a plain copy kernel, and
+    as the measured stream-copy bandwidth).  Returns the best achieved bandwidth
+    (bytes read + written per second) over `reps` timed copies."""
+    ctypes, hip = _hip()
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
         raise RuntimeError("hipMalloc failed")
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
     hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
     hip.hipDeviceSynchronize()
-    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
-    hip.hipDeviceSynchronize()
+    best = 0.0
+    for _ in range(reps):
+        hip.hipEventRecord(e0, None)
+        hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), 3, None)  # hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        if ms.value > 0:
+            best = max(best, 2 * nbytes / (ms.value * 1e-3))
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
     hip.hipFree(a)
     hip.hipFree(b)
+    return best
 
 
 def measured_traffic(config, kernel):
@@ -86,33 +109,57 @@ def measured_traffic(config, kernel):
     return None if k is None else k.get("bytes_per_launch")
 
 
-def cpu_baseline(w, titer):
-    """FP32 NumPy port of the same solver (the oracle in float32) on a bounded sample:
-    one slice of the workload (all projections, all basis functions), one inner step
-    and one prox/dual update, single thread.  Scaled to voxel-iter/s of a titer-step
-    outer iteration."""
+CPU_THREADS = 16  # the host share of one GPU on the box
+
+
+def cpu_baseline(w, titer, threads=None):
+    """FP32 NumPy/SciPy port of the same solver (the oracle in float32) on a bounded
+    sample of the workload, on `threads` host threads (default min(16, cores)):
+    `threads` slices of one chunk, each thread running the reference's per-chunk
+    ADMM step (admm.cu:74-122) on its own one-slice chunk (halo slabs from the
+    previous iterate, all projections, all basis functions): once with one inner
+    step and once with none (prox / dual update only).  The chunk loop is the
+    reference's own unit of parallel work (one GPU per chunk, rectv.cu:161-162), so
+    the time of an outer iteration scales as titer*inner + prox per slice.
+    value = n^2*M*slices / (titer*t_inner + t_prox)."""
+    import threading
     from oracle import rectv_oracle as O
     O.set_workers(1)
+    threads = threads or min(CPU_THREADS, os.cpu_count() or 1)
     n, nt, m = w["n"], w["ntheta"], w["m"]
     dt = np.float32
     ops = O.Operators(n, nt, m, w["theta"], w["phi"], w["center"], w["lambda1"], dt)
-    g = w["data"][:1].astype(dt)
-    fext = np.zeros((3, m, n, n), dt)
-    fext[1] = np.random.default_rng(0).random((m, n, n)).astype(dt) * 0.1
-    psi = np.zeros((2, m + 1, n + 1, n + 1, 4), dt)
-    mu = np.zeros_like(psi)
-    t0 = time.perf_counter()
-    O.solver_admm(ops, fext, g, psi, mu, w["lambda0"] * w["lambda1"], 0.5, 0, 0, 1)
-    t1 = time.perf_counter()
-    O.solver_admm(ops, fext, g, psi, mu, w["lambda0"] * w["lambda1"], 0.5, 0, 0, 0)
-    t2 = time.perf_counter()
-    t_prox = t2 - t1
-    t_inner = max(t1 - t0 - t_prox, 1e-9)
+    lam0 = w["lambda0"] * w["lambda1"]
+    rng = np.random.default_rng(0)
+    f0 = rng.random((threads + 2, m, n, n)).astype(dt) * 0.1
+
+    def one(z, t_iter):
+        fext = f0[z:z + 3].copy()
+        g = w["data"][z % w["nz"]][None].astype(dt)
+        psi = np.zeros((2, m + 1, n + 1, n + 1, 4), dt)
+        mu = np.zeros_like(psi)
+        O.solver_admm(ops, fext, g, psi, mu, lam0, 0.5, 1, 1, t_iter)
+
+    def timed(t_iter):
+        ths = [threading.Thread(target=one, args=(z, t_iter)) for z in range(threads)]
+        t0 = time.perf_counter()
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        return time.perf_counter() - t0
+
+    t_step = timed(1)
+    t_prox = timed(0)
+    t_inner = max(t_step - t_prox, 1e-9)
     t_iter = titer * t_inner + t_prox
-    return {"value": n * n * m / t_iter, "unit": "voxel-iter/s", "cores": 1, "kind": "port",
-            "sample": "1 slice (n=%d, %d projections, M=%d): one inner ADMM step %.2f s + prox %.2f s, "
-                      "FP32 NumPy/SciPy, 1 thread; value = n^2*M / (titer*inner + prox) with titer=%d"
-                      % (n, nt, m, t_inner, t_prox, titer)}
+    return {"value": n * n * m * threads / t_iter, "unit": "voxel-iter/s", "cores": threads, "kind": "port",
+            "sample": "%d slices of one chunk (n=%d, %d projections, M=%d), one thread per slice running the "
+                      "per-chunk ADMM step on a one-slice chunk: one inner step %.2f s + prox/dual update %.2f s "
+                      "(wall, all threads), FP32 NumPy/SciPy on %d host threads (os.cpu_count() = %s); "
+                      "value = n^2*M*slices / (titer*inner + prox) with titer=%d, i.e. extrapolated from one "
+                      "slice per thread to the whole volume" % (threads, n, nt, m, t_inner, t_prox, threads,
+                                                                os.cpu_count(), titer)}
 
 
 def batched_bench(count, niter, titer, workers=8):
@@ -207,7 +254,8 @@ def main():
     ap.add_argument("--config", default="full")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--copy-probe", action="store_true", help="also run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="accepted for older profiling commands: the 1 GiB device copy (counter calibration, copy bandwidth) always runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default gather/scatter launch shapes")
     ap.add_argument("--mode", default="all", choices=["all", "plain", "batched", "served"],
                     help="plain: the headline solve only; batched / served: those forms only; all: each")
@@ -288,7 +336,9 @@ def main():
             "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4),
             "traffic": measured_traffic(a.config, dom), "alg_bytes_per_launch": db}
     B = byte_model(n, nt, m, nz, nzp, a.titer)
+    Bp = byte_model(n, nt, m, nz, nzp, a.titer, pairs=True)
     s.free()
+    copy_bw = copy_probe()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "voxel-iter/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
@@ -297,13 +347,18 @@ def main():
                    % (n, nz, nzp, nt, w["nrot"], w["ntheta_rot"], m, a.titer),
                    "n": n, "nz": nz, "nzp": nzp, "ntheta": nt, "m": m, "titer": a.titer,
                    "launch_shapes": None if tuned is None else {"gather_slices": tuned[0], "scatter_slices": tuned[1]}},
+        # byte model of the reference algorithm (M*Nz spectra) and of the frame-pair
+        # pipeline this solver runs (ceil(M/2)*Nz spectra), against the 8 TB/s peak
+        # and against the measured device-to-device copy bandwidth
         "iteration_hbm_frac": round(B / (ms * 1e-3) / HBM_PEAK, 4),
         "iteration_model_bytes": B,
+        "iteration_hbm_frac_pairs": round(Bp / (ms * 1e-3) / HBM_PEAK, 4),
+        "iteration_model_bytes_pairs": Bp,
+        "copy_bandwidth_gbps": round(copy_bw / 1e9, 1),
+        "iteration_copy_frac_pairs": round(Bp / (ms * 1e-3) / copy_bw, 4) if copy_bw > 0 else None,
         "roofline": roof,
         "kernels": kern,
     }
-    if a.copy_probe:
-        copy_probe()
     if a.mode == "all":
         out["batched"] = batched_bench(a.jobs, a.job_niter, a.job_titer)
         out["served"] = served_bench(a.jobs, a.job_niter, a.job_titer, a.rate)

@@ -29,7 +29,17 @@ Pinning (what makes this oracle trustworthy, see tests/test_oracle_pins.py):
   * ADMM: invariants (lambda0=0 keeps mu at 0, niter=0 returns zeros), descent of
     the data residual on a noise-free problem.
   * check_adjoints: res[0] == res[1] to FP64 rounding.
-No function here is "parity unpinned".
+  * solve level (tests/test_oracle_solve_pins.py): basis orientation (with M =
+    ntheta the takephi basis is a full DFT, so row k of R_apr f is sqrt(M) Re
+    fwdR(f_k) row k), frame times t_i against hand-evaluated C integer division,
+    chunk-order invariance of run, the rho rule's direction against residuals
+    computed from the iterates (residual balancing), and the fixed-point
+    conditions of a converged run (psi = grad f, mu in the group subdifferential,
+    R^*(Rf - g) + div mu = 0).  tests/test_oracle_mutations.py plants five
+    plausible mistakes (time-reversed basis, inverted rho rule, dual-update sign,
+    halo from the new iterate, rounded frame times) in copies of this module and
+    checks that each fails one of these pins.
+Every function here has at least one of these checks.
 
 Precision: FP64 throughout by default (``dtype=np.float64``).  With
 ``dtype=np.float32`` the same code runs in FP32/complex64; that variant is the
@@ -372,23 +382,35 @@ def solve_reg(h2, mu, lam, rho):
     return out.astype(h2.dtype)
 
 
-def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer):
+def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer, stats=None):
     """One chunk's ADMM step, admm.cu:74-122.
 
     fext [nzp+2, m, n, n] is the working copy fm with halo slabs; it is updated in
     place.  psi, mu [nzp+1, m+1, n+1, n+1, 4] are updated in place.  Returns
     (||h2 - psi||, ||grad f_old - grad f_new||) of admm.cu:115-117.
+
+    ``stats`` (a dict, optional) receives the two terms of the objective that the
+    step evaluates on the way: ``data`` = 1/2 ||R fm - g||^2 of the chunk as it
+    enters (the first inner step's residual h1; computed directly when titer = 0)
+    and ``tv`` = sum over dual positions of the 4-vector norm of the gradient h2 of
+    the updated chunk (the h2 of admm.cu:101, halos from the previous iterate).
     """
     inner = fext[1:-1]
     hs = ops.gradient(fext, flgl, flgr)                         # h2stored = grad fm (:79)
-    for _ in range(titer):
+    for k in range(titer):
         h2 = ops.gradient(fext, flgl, flgr)                     # (:85)
         h1 = ops.radonapr(inner, 1.0)                           # (:87)
         h2 = rho * h2 - rho * psi + mu                          # lin4 (:90)
         h1 = h1 - g                                             # lin (:92)
+        if k == 0 and stats is not None:
+            stats["data"] = 0.5 * float(np.sum(h1.astype(np.float64) ** 2))
         ops.divergent(inner, h2, -0.5)                          # (:95)
         ops.radonapradj(inner, h1, -0.5)                        # (:97)
+    if titer == 0 and stats is not None:
+        stats["data"] = 0.5 * float(np.sum((ops.radonapr(inner, 1.0) - g).astype(np.float64) ** 2))
     h2 = ops.gradient(fext, flgl, flgr)                         # (:101)
+    if stats is not None:
+        stats["tv"] = float(np.sum(np.sqrt(np.sum(h2.astype(np.float64) ** 2, axis=-1))))
     psi[...] = solve_reg(h2, mu, lambda0, rho)                  # (:104)
     mu += rho * h2 - rho * psi                                  # (:107)
     hs = hs - h2                                                # (:109)
@@ -397,13 +419,29 @@ def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer):
 
 
 def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter, titer,
-        dtype=np.float64, log=None, history=None):
+        dtype=np.float64, log=None, history=None, order=None, state=None):
     """rectv::run, rectv.cu:102-252 (lambda0 arrives already multiplied by lambda1).
 
     Chunks of nzp slices are processed against the previous outer iterate f (halo
     slabs included) and write fn; then rho is rebalanced (r > 10 s -> rho*2,
     s > 10 r -> rho/2, rectv.cu:224-232), ||fn - f||^2 is reported and f/fn swap.
-    ``history`` (a list) receives one dict per outer iteration.
+
+    ``history`` (a list) receives one dict per outer iteration: r, s, rho used,
+    rho for the next iteration, rdiff = ||fn - f||^2, ||fn||^2, and the objective
+    terms ``data`` = 1/2 ||R_apr f - g||^2 of the iterate entering the iteration
+    and ``tv`` = sum of the dual-position gradient norms of the iterate leaving it
+    (per chunk, halos from the entering iterate, ghost positions included; see
+    solver_admm).  The fixed point of the iteration minimises (up to the ghost
+    positions) J(f) = 1/2 ||R_apr f - g||^2 + lambda0/2 * sum ||grad f||: the inner
+    step descends 1/2||R f - g||^2 + 1/2 (<mu, grad f - psi> + rho/2 ||grad f - psi||^2)
+    because div is half the transpose of grad, i.e. ADMM with penalty rho/2 on
+    lambda0/2 ||psi||_{2,1}.
+    ``order`` (a permutation of the chunk indices, default ascending) sets the
+    order in which chunks are processed; the reference reads f and writes fn, so
+    the result does not depend on it (rectv.cu:170-171).
+    ``state`` (a dict, optional): entries f, psi, mu, rho present on entry replace
+    the reference's zero start (rectv.cu:119-125); on return it holds the final
+    f, psi, mu and rho.
     Returns f [nz, m, n, n].
     """
     if nz % nzp != 0:
@@ -412,14 +450,20 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
     nch = nz // nzp
     ops = Operators(n, ntheta, m, theta, phi, center, lambda1, dt)
     g = np.asarray(data, dtype=np.float32).astype(dt).reshape(nz, ntheta, n)
-    f = np.zeros((nz, m, n, n), dt)
+    st = {} if state is None else state
+    f = np.array(st["f"], dt) if "f" in st else np.zeros((nz, m, n, n), dt)
     fn = np.zeros_like(f)
-    psi = np.zeros((nch, nzp + 1, m + 1, n + 1, n + 1, 4), dt)
-    mu = np.zeros_like(psi)
-    rho = 0.5
+    shp = (nch, nzp + 1, m + 1, n + 1, n + 1, 4)
+    psi = np.array(st["psi"], dt).reshape(shp) if "psi" in st else np.zeros(shp, dt)
+    mu = np.array(st["mu"], dt).reshape(shp) if "mu" in st else np.zeros(shp, dt)
+    rho = float(st.get("rho", 0.5))
+    chunks = list(range(nch)) if order is None else [int(c) for c in order]
+    if sorted(chunks) != list(range(nch)):
+        raise ValueError("order must be a permutation of the chunk indices")
     for it in range(niter):
-        nd = []
-        for iz in range(nch):
+        nd = [None] * nch
+        data_t = tv_t = 0.0
+        for iz in chunks:
             flgl, flgr = int(iz != 0), int(iz != nch - 1)
             fext = np.zeros((nzp + 2, m, n, n), dt)
             lo = iz * nzp
@@ -428,8 +472,11 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
                 fext[0] = f[lo - 1]
             if flgr:
                 fext[-1] = f[lo + nzp]
-            nd.append(solver_admm(ops, fext, g[lo:lo + nzp], psi[iz], mu[iz],
-                                  lambda0, rho, flgl, flgr, titer))
+            sts = {}
+            nd[iz] = solver_admm(ops, fext, g[lo:lo + nzp], psi[iz], mu[iz],
+                                 lambda0, rho, flgl, flgr, titer, stats=sts)
+            data_t += sts["data"]
+            tv_t += sts["tv"]
             fn[lo:lo + nzp] = fext[1:-1]
         r = sum(x * x for x, _ in nd)
         s = sum(rho * rho * y * y for _, y in nd)
@@ -442,9 +489,21 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
         if log is not None:
             print("iter (%d/%d), rdiff ||f(k+1)-f(k)||:%f " % (it, niter, norm), file=log)
         if history is not None:
-            history.append(dict(iter=it, r=r, s=s, rho=rho_used, rho_next=rho, rdiff=norm))
+            history.append(dict(iter=it, r=r, s=s, rho=rho_used, rho_next=rho, rdiff=norm,
+                                fnorm=float(np.sum(fn.astype(np.float64) ** 2)), data=data_t, tv=tv_t))
         f, fn = fn, f
+    if state is not None:
+        state.update(f=f, psi=psi, mu=mu, rho=rho)
     return f
+
+
+def rho_margin(r, s):
+    """Distance of an outer iteration's (r, s) from the two thresholds of the rho
+    rule (rectv.cu:231-232) as min(|log(r / 10 s)|, |log(s / 10 r)|): near zero, FP32
+    and FP64 arithmetic may take different branches (inf when r or s is zero)."""
+    if r <= 0 or s <= 0:
+        return np.inf
+    return float(min(abs(np.log(r / (10 * s))), abs(np.log(s / (10 * r)))))
 
 
 def check_adjoints(data, theta, phi, n, ntheta, m, nz, nzp, lambda1, center, dtype=np.float64):
@@ -485,10 +544,11 @@ def check_adjoints(data, theta, phi, n, ntheta, m, nz, nzp, lambda1, center, dty
 def takephi(ntheta, m):
     """Fourier temporal basis, tests/test_adjoint.py:23-27 and foam_example.py:17-23.
 
-    phi_j[k] = exp(2 pi i (k - ntheta/2)(j - m/2)/ntheta)/sqrt(ntheta), stored as
-    float32 [m, 2*ntheta] with re/im interleaved.
+    phi_j[k] = exp(2 pi i a_k b_j/ntheta)/sqrt(ntheta) with a = arange(-ntheta//2,
+    ntheta//2), b = arange(-m//2, m//2) (Python floor division, so odd sizes start
+    at -(size+1)/2), stored as float32 [m, 2*ntheta] with re/im interleaved.
     """
-    x, y = np.meshgrid(np.arange(ntheta) - ntheta // 2, np.arange(m) - m // 2)
+    x, y = np.meshgrid(np.arange(-ntheta // 2, ntheta // 2), np.arange(-m // 2, m // 2))
     phi = np.zeros([m, 2 * ntheta], dtype='float32')
     phi[:, ::2] = np.cos(2 * np.pi * x * y / ntheta) / np.sqrt(ntheta)
     phi[:, 1::2] = np.sin(2 * np.pi * x * y / ntheta) / np.sqrt(ntheta)

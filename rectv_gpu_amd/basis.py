@@ -4,10 +4,13 @@ import numpy as np
 
 def takephi(ntheta, m):
     """Fourier temporal basis as float32 [m, 2*ntheta] (re/im interleaved):
-    phi_j[k] = exp(2 pi i (k - ntheta/2)(j - m/2)/ntheta)/sqrt(ntheta)
-    (tests/test_adjoint.py:23-27, examples/foam/foam_example.py:17-23)."""
-    k = np.arange(ntheta) - ntheta // 2
-    j = np.arange(m) - m // 2
+    phi_j[k] = exp(2 pi i a_k b_j / ntheta)/sqrt(ntheta) with a = -ntheta//2, ...,
+    ntheta//2 - 1 and b = -m//2, ..., m//2 - 1 in Python floor division, i.e.
+    a_k = k + floor(-ntheta/2) (tests/test_adjoint.py:23-27,
+    examples/foam/foam_example.py:17-23; for odd sizes the ranges start at
+    -(size+1)/2)."""
+    k = np.arange(-ntheta // 2, ntheta // 2)
+    j = np.arange(-m // 2, m // 2)
     x, y = np.meshgrid(k, j)
     phi = np.zeros([m, 2 * ntheta], dtype='float32')
     phi[:, ::2] = np.cos(2 * np.pi * x * y / ntheta) / np.sqrt(ntheta)

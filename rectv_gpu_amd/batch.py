@@ -271,33 +271,46 @@ class Server:
                 continue
             job.t_start = time.perf_counter()
             k = min(self.ahead, job.niter)
-            if k > 0:
-                s.iterate_async(k, job.titer, job.lambda0 * job.lambda1, job.lambda1)
+            try:
+                if k > 0:
+                    s.iterate_async(k, job.titer, job.lambda0 * job.lambda1, job.lambda1)
+            except Exception as e:  # a failing enqueue fails its job only
+                self._fail(job, s, nb, e)
+                continue
             self._active.append([job, s, nb, k, 0])
         self.peak_active = max(self.peak_active, len(self._active))
         keep = []
         for ent in self._active:
             job, s, nb, enq, chk = ent
-            if not s.idle():
-                keep.append(ent)
-                continue
-            progressed = True
-            hist = s.history(chk, enq - chk) if enq > chk else np.zeros((0, 5))
-            ent[4] = enq
-            if enq >= job.niter or _converged(job, hist):
-                try:
+            try:
+                if not s.idle():
+                    keep.append(ent)
+                    continue
+                progressed = True
+                hist = s.history(chk, enq - chk) if enq > chk else np.zeros((0, 5))
+                ent[4] = enq
+                if enq >= job.niter or _converged(job, hist):
                     finish(job, s)
-                except Exception as e:
-                    job.error = str(e)
-                    s.free()
-                self._retire(job, nb)
+                    self._retire(job, nb)
+                    continue
+                k = min(self.ahead, job.niter - enq)
+                s.iterate_async(k, job.titer, job.lambda0 * job.lambda1, job.lambda1)
+            except Exception as e:  # an error on one job's solver ends that job only
+                progressed = True
+                self._fail(job, s, nb, e)
                 continue
-            k = min(self.ahead, job.niter - enq)
-            s.iterate_async(k, job.titer, job.lambda0 * job.lambda1, job.lambda1)
             ent[3] = enq + k
             keep.append(ent)
         self._active = keep
         return progressed
+
+    def _fail(self, job, s, nbytes, err):
+        job.error = str(err)
+        try:
+            s.free()
+        except Exception:
+            pass
+        self._retire(job, nbytes)
 
     def run(self, until=None, poll=2e-4):
         """Drive the loop until ``until()`` is true (default: nothing pending)."""

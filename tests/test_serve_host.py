@@ -175,3 +175,26 @@ def test_autotune_policy():
         s.free()
         assert Tracking.tuned == want, (job.n, mode)
 
+
+
+def test_enqueue_failure_fails_that_job_only():
+    """A solver error while iterating (e.g. a HIP error reported by iterate_async)
+    ends that job with its error and frees its memory; the others finish."""
+
+    class Flaky(FakeSolver):
+        def iterate_async(self, k, titer, l0, l1):
+            if self.job.name == "flaky" and self.enq >= 2:
+                raise RuntimeError("device error in enqueue")
+            super().iterate_async(k, titer, l0, l1)
+
+    FakeSolver.live = FakeSolver.peak = 0
+    srv = Server(factory=lambda j: Flaky(j), footprint=lambda j: j.fp, budget=100, max_active=3, ahead=1)
+    ok = [srv.submit(_job(niter=4)) for _ in range(2)]
+    fl = _job(niter=6)
+    fl.name = "flaky"
+    srv.submit(fl)
+    srv.run()
+    srv.close()
+    assert all(j.error is None and j.iterations == 4 for j in ok)
+    assert fl.error and "enqueue" in fl.error and fl.done.is_set()
+    assert srv.used == 0 and FakeSolver.live == 0
