@@ -70,6 +70,30 @@ int rtv_iterate(rtv_solver *h, int niter, int titer, float lambda0, float lambda
 int rtv_get_result(rtv_solver *h, float *fres);
 int rtv_synchronize(rtv_solver *h);
 
+/* ---- solver state (checkpoint / restore).  No reference counterpart: rectv::run
+ * always starts from zero (src/cuda/rectv.cu:119-125).  Layouts: f [nz][m][n][n]
+ * float32 (the previous iterate, from which halo slabs are read);
+ * psi, mu [nz/nzp][nzp+1][m+1][n+1][n+1][4] float32 (the reference's float4 dual
+ * arrays per chunk, components x, y, t, z, ghost entries included);
+ * dchi = div(mu - rho*psi) [nz][m][n][n]. */
+/* Sets the state (NULL f / psi / mu: zero), forms div(mu - rho psi) for lambda1 and
+ * restarts the iteration counter and history; rtv_iterate then continues from it. */
+int rtv_set_state(rtv_solver *h, const float *f, const float *psi, const float *mu, float rho, float lambda1);
+/* Copies the state out (any pointer may be NULL; rho: 1 float). */
+int rtv_get_state(rtv_solver *h, float *f, float *psi, float *mu, float *dchi, float *rho);
+
+/* ---- single steps of the ADMM iteration on the resident state (parity tests of
+ * the fused kernels; src/cuda/admm.cu:74-122 split at its two phases). */
+/* One inner step (admm.cu:85-97) of every chunk applied to fcur [nz][m][n][n]:
+ * fout = fcur - 1/2 div(rho(grad fcur - psi) + mu) - 1/2 R_apr^*(R_apr fcur - g),
+ * halo slabs from the resident previous iterate; rtv_set_inputs supplies g. */
+int rtv_apply_inner_step(rtv_solver *h, const float *fcur, float *fout, float lambda1);
+/* The closing phase (admm.cu:101-117 with the rho rule of rectv.cu:224-232) applied
+ * to fcur as the new iterate: updates the resident psi, mu, rho and
+ * div(mu - rho psi); row (may be NULL) receives the history row (7 doubles, see
+ * rtv_history).  The resident previous iterate is unchanged. */
+int rtv_apply_prox(rtv_solver *h, const float *fcur, float lambda0, float lambda1, double *row);
+
 /* ---- operator entry points (parity tests).  Each applies one reference operator to
  * all nz slices, with inputs/outputs in host memory; rtv_set_inputs must have
  * been called (it supplies theta, phi and center). */
@@ -103,8 +127,12 @@ int rtv_iterate_async(rtv_solver *h, int niter, int titer, float lambda0, float 
 int rtv_query(rtv_solver *h, int *idle);
 /* Outer iterations enqueued since the last rtv_reset. */
 int rtv_iterations(rtv_solver *h, long long *count);
-/* Rows [first, first+count) since reset (waits for the solver's stream), 5 doubles
- * each: r, s, rho used, ||f(k+1)-f(k)||^2, ||f(k+1)||^2.  The last 1024 rows are held. */
+/* Rows [first, first+count) since reset (waits for the solver's stream), 7 doubles
+ * each: r, s, rho used, ||f(k+1)-f(k)||^2, ||f(k+1)||^2, 1/2||R_apr f(k) - g||^2 (the
+ * data term of the iterate entering iteration k; NaN when titer = 0), and
+ * sum |grad f(k+1)| over the dual positions of each chunk (halos from f(k)).  The
+ * objective 1/2||R f - g||^2 + lambda0/2 sum|grad f| of iterate k is column 5 of
+ * row k plus lambda0/2 times column 6 of row k-1.  The last 1024 rows are held. */
 int rtv_history(rtv_solver *h, long long first, int count, double *rows);
 /* on != 0: each outer iteration is captured once as a HIP graph and replayed. */
 int rtv_set_graph(rtv_solver *h, int on);

@@ -419,7 +419,7 @@ def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer, stats=No
 
 
 def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter, titer,
-        dtype=np.float64, log=None, history=None, order=None, state=None):
+        dtype=np.float64, log=None, history=None, order=None, state=None, threads=1):
     """rectv::run, rectv.cu:102-252 (lambda0 arrives already multiplied by lambda1).
 
     Chunks of nzp slices are processed against the previous outer iterate f (halo
@@ -442,6 +442,8 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
     ``state`` (a dict, optional): entries f, psi, mu, rho present on entry replace
     the reference's zero start (rectv.cu:119-125); on return it holds the final
     f, psi, mu and rho.
+    ``threads`` > 1 processes that many chunks at a time on host threads (chunks
+    read f and write disjoint parts of fn, psi and mu, so the result is the same).
     Returns f [nz, m, n, n].
     """
     if nz % nzp != 0:
@@ -460,10 +462,15 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
     chunks = list(range(nch)) if order is None else [int(c) for c in order]
     if sorted(chunks) != list(range(nch)):
         raise ValueError("order must be a permutation of the chunk indices")
+    pool = None
+    if threads > 1:
+        import concurrent.futures as cf
+        pool = cf.ThreadPoolExecutor(max_workers=threads)
     for it in range(niter):
         nd = [None] * nch
-        data_t = tv_t = 0.0
-        for iz in chunks:
+        sts = [dict() for _ in range(nch)]
+
+        def chunk(iz):
             flgl, flgr = int(iz != 0), int(iz != nch - 1)
             fext = np.zeros((nzp + 2, m, n, n), dt)
             lo = iz * nzp
@@ -472,12 +479,17 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
                 fext[0] = f[lo - 1]
             if flgr:
                 fext[-1] = f[lo + nzp]
-            sts = {}
             nd[iz] = solver_admm(ops, fext, g[lo:lo + nzp], psi[iz], mu[iz],
-                                 lambda0, rho, flgl, flgr, titer, stats=sts)
-            data_t += sts["data"]
-            tv_t += sts["tv"]
+                                 lambda0, rho, flgl, flgr, titer, stats=sts[iz])
             fn[lo:lo + nzp] = fext[1:-1]
+
+        if pool is None:
+            for iz in chunks:
+                chunk(iz)
+        else:
+            list(pool.map(chunk, chunks))
+        data_t = sum(x["data"] for x in sts)
+        tv_t = sum(x["tv"] for x in sts)
         r = sum(x * x for x, _ in nd)
         s = sum(rho * rho * y * y for _, y in nd)
         rho_used = rho
@@ -492,6 +504,8 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
             history.append(dict(iter=it, r=r, s=s, rho=rho_used, rho_next=rho, rdiff=norm,
                                 fnorm=float(np.sum(fn.astype(np.float64) ** 2)), data=data_t, tv=tv_t))
         f, fn = fn, f
+    if pool is not None:
+        pool.shutdown()
     if state is not None:
         state.update(f=f, psi=psi, mu=mu, rho=rho)
     return f

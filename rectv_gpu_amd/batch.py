@@ -61,7 +61,7 @@ class Job:
         Job._ids += 1
         self.name = name if name is not None else "job%d" % Job._ids
         self.result = None
-        self.history = None      # [iterations, 5]: r, s, rho, ||df||^2, ||f||^2
+        self.history = None      # [iterations, 7]: r, s, rho, ||df||^2, ||f||^2, data term, sum |grad f|
         self.iterations = 0
         self.error = None
         self.t_submit = self.t_admit = self.t_start = self.t_done = None
@@ -287,7 +287,7 @@ class Server:
                     keep.append(ent)
                     continue
                 progressed = True
-                hist = s.history(chk, enq - chk) if enq > chk else np.zeros((0, 5))
+                hist = s.history(chk, enq - chk) if enq > chk else np.zeros((0, 7))
                 ent[4] = enq
                 if enq >= job.niter or _converged(job, hist):
                     finish(job, s)

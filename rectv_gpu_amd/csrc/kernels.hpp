@@ -1176,7 +1176,8 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 // P1: after the inner steps: h2 = grad fm; psi = shrink(h2 + mu/rho, lam0/rho);
 // mu += rho (h2 - psi); partial sums r = |h2 - psi|^2, s = |grad(f_old - fm)|^2
 // and ||fm - f_old||^2 (admm.cu:101-117, rectv.cu:224-237), plus ||fm||^2 (the
-// scale of the relative change used by callers that stop early), over all dual
+// scale of the relative change used by callers that stop early) and the sum of
+// the group norms |h2| (the penalty term of the objective), over all dual
 // entries, ghost entries included.  One thread per dual entry, no state carried
 // between entries: workgroup = (chunk, z', t') plane and a band of PX_ROWS rows
 // y'; wave w takes rows y0 + w, y0 + w + PX_WAVES, ..., lanes take x' = lane + 64 k.
@@ -1185,7 +1186,7 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 // mostly cache hits: blocks run plane after plane over one band of rows, so the +t
 // and +z planes are read again by the next blocks).  Halo slabs come from A
 // (rectv.cu:170-171).
-constexpr int PROX_W = 4;  // partial sums per workgroup
+constexpr int PROX_W = 5;  // partial sums per workgroup: r, s, ||fm - f_old||^2, ||fm||^2, sum |h2|
 constexpr int PX_ROWS = 16;  // rows y' per workgroup
 #ifndef RTV_PX_WAVES
 #define RTV_PX_WAVES 16
@@ -1225,7 +1226,7 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
     const size_t sc0 = slab(zq, hc) * S, sn = slab(zq + 1, hn) * S;
     const int ic = iq > 0 ? iq - 1 : 0, it = iq < M ? iq : M - 1;
     const size_t fc = (size_t)ic * plane, ft = (size_t)it * plane;
-    double sr = 0.0, ss = 0.0, sc = 0.0, sf = 0.0;
+    double sr = 0.0, ss = 0.0, sc = 0.0, sf = 0.0, st = 0.0;
     const int y1 = min(n + 1, (band + 1) * PX_ROWS);
     for (int yq = band * PX_ROWS + w; yq < y1; yq += PX_WAVES) {
         const int yc = yq > 0 ? yq - 1 : 0, yn = yq < n ? yq : n - 1;
@@ -1249,6 +1250,7 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
             const float dx = a.gs * (xD - d0), dy = a.gs * (yD - d0);
             const float dt = a.gt * ((iq < M ? tD : d0) - d0), dz = a.gs * (zD - d0);
             ss += (double)dx * dx + (double)dy * dy + (double)dt * dt + (double)dz * dz;
+            st += (double)sqrtf(hx * hx + hy * hy + ht * ht + hz * hz);
             if (zq >= 1 && iq >= 1 && yq >= 1 && xq >= 1) {
                 sc += (double)d0 * d0;
                 sf += (double)e0 * e0;
@@ -1284,12 +1286,14 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
         ss += __shfl_down(ss, o, 64);
         sc += __shfl_down(sc, o, 64);
         sf += __shfl_down(sf, o, 64);
+        st += __shfl_down(st, o, 64);
     }
     if (lane == 0) {
         red[0][w] = sr;
         red[1][w] = ss;
         red[2][w] = sc;
         red[3][w] = sf;
+        red[4][w] = st;
     }
     __syncthreads();
     if (threadIdx.x == 0)
@@ -1303,21 +1307,26 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
 // The penalty rule between outer iterations (rectv.cu:220-232), on the device so
 // that an outer iteration needs no host round trip (and can be replayed as one
 // graph): with red = {sum |h2 - psi|^2, sum |grad(f_old - fm)|^2, ||fn - f||^2,
-// ||fn||^2}, r = red[0], s = rho^2 red[1]; rho *= 2 if r > 10 s, rho /= 2 if
-// s > 10 r.  Row *iter of hist (capacity cap rows, wrapping) receives
-// {r, s, rho used, ||fn - f||^2, ||fn||^2}; *iter is then incremented.
+// ||fn||^2, sum |h2|, ||R f - g||^2}, r = red[0], s = rho^2 red[1]; rho *= 2 if
+// r > 10 s, rho /= 2 if s > 10 r.  Row *iter of hist (capacity cap rows of
+// HIST_W, wrapping) receives {r, s, rho used, ||fn - f||^2, ||fn||^2,
+// 1/2 ||R f - g||^2 of the entering iterate (NaN when no inner step ran),
+// sum |grad fn|}; *iter is then incremented.
+constexpr int HIST_W = 7;
 __global__ void k_rho_rule(const double *__restrict__ red, float *__restrict__ rho, double *__restrict__ hist,
-                           int *__restrict__ iter, int cap) {
+                           int *__restrict__ iter, int cap, int has_data) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const float rh = *rho;
     const double r = red[0], s = (double)rh * rh * red[1];
     const int it = *iter;
-    double *h = hist + (size_t)(it % cap) * 5;
+    double *h = hist + (size_t)(it % cap) * HIST_W;
     h[0] = r;
     h[1] = s;
     h[2] = rh;
     h[3] = red[2];
     h[4] = red[3];
+    h[5] = has_data ? 0.5 * red[5] : __builtin_nan("");
+    h[6] = red[4];
     float nr = rh;
     if (r > 10 * s) nr = rh * 2;
     else if (s > 10 * r) nr = rh * 0.5f;

@@ -459,7 +459,7 @@ struct Solver {
     // penalty rho, its rule and the per-iteration norms live on the device
     // (k_rho_rule), so an outer iteration needs no host round trip
     DBuf<float> d_rho;
-    DBuf<double> d_hist;  // [HIST_CAP][5] ring: r, s, rho used, ||fn - f||^2, ||fn||^2
+    DBuf<double> d_hist;  // [HIST_CAP][HIST_W] ring: r, s, rho used, ||fn - f||^2, ||fn||^2, data term, sum |grad fn|
     DBuf<int> d_iter;     // rows written since reset
     static constexpr int HIST_CAP = 1024;
     long long iters_done = 0;  // outer iterations enqueued since reset
@@ -475,6 +475,7 @@ struct Solver {
     DBuf<float2> d_H, d_G, d_v, d_gq, d_q;
     DBuf<float> d_h1;
     DBuf<double> d_part, d_red;
+    DBuf<double> d_rpart;  // per ray workgroup: ||R f - g||^2 partials of the first inner step
     // geometry
     bool inputs = false;
     DBuf<SampleGeom> d_geo;
@@ -534,7 +535,10 @@ struct Solver {
         const size_t geom = ni * (16 + 16 + 64 + 32 + 64 + 9) + nchk * (8 + 16 * 8 + 16 * 3 + 16 + SM_CH * 128) +
                             3 * ntile * 16 +
                             (size_t)nt * 2 * MP * 8 + 8 * (size_t)n * 8 + 2 * (N + n) * 8 + N * 8;
-        return state + per_slice * nz + geom + (1 << 20);
+        // partial sums of the prox (per workgroup) and of the ray passes (per ray, at most)
+        const size_t pb = (size_t)((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * nch;
+        const size_t parts = (PROX_W * (pb + (size_t)nz * nt) + 16 + (size_t)nz * nt + 1) * 8;
+        return state + per_slice * nz + geom + parts + (1 << 20);
     }
 
     void init_device() {
@@ -580,9 +584,10 @@ struct Solver {
         const size_t pb = (size_t)((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * nch;  // prox workgroups
         const size_t rb = (size_t)((zb * nt + T_ray - 1) / T_ray);
         d_part.alloc(PROX_W * std::max(pb, rb) + 16, bytes);
+        d_rpart.alloc((size_t)((nch + cb - 1) / cb) * rb + 1, bytes);
         d_red.alloc(8, bytes);
         d_rho.alloc(1, bytes);
-        d_hist.alloc((size_t)HIST_CAP * 5, bytes);
+        d_hist.alloc((size_t)HIST_CAP * HIST_W, bytes);
         d_iter.alloc(1, bytes);
         RTV_HIP(hipStreamSynchronize(st));
         if (std::getenv("RECTV_VERBOSE"))
@@ -819,9 +824,10 @@ struct Solver {
         else scatter_t<4>(zc);
     }
 
+    int ray_blocks(int zc) const { return (zc * nt + T_ray - 1) / T_ray; }
     void ray(int zc, const float *gdat, const float *h1in, float *h1out, int mode, double *partial) {
         const int nrays = zc * nt;
-        const int blocks = (nrays + T_ray - 1) / T_ray;
+        const int blocks = ray_blocks(zc);
         const int thr = fft_threads(T_ray, n);
         const size_t lds = (size_t)T_ray * st_n * sizeof(float2);
         launch("ray", [&] {
@@ -887,7 +893,7 @@ struct Solver {
         else if (M <= 16) launch_update<16>(cur, A, nxt, ta, ncb, zv0);
         else launch_update<32>(cur, A, nxt, ta, ncb, zv0);
     }
-    void prox(const float *cur, const float *A, const TvArgs &ta) {
+    void prox(const float *cur, const float *A, const TvArgs &ta, bool has_data) {
         const dim3 grid(((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * nch);
         launch("prox", [&] {
             hipLaunchKernelGGL(k_prox, grid, dim3(64 * PX_WAVES), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
@@ -895,7 +901,8 @@ struct Solver {
         const int cnt = grid.x;
         launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, PROX_W); });
         launch("rho", [&] {
-            hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, st, d_red.p, d_rho.p, d_hist.p, d_iter.p, HIST_CAP);
+            hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, st, d_red.p, d_rho.p, d_hist.p, d_iter.p, HIST_CAP,
+                               has_data ? 1 : 0);
         });
     }
     template <int MM>
@@ -934,21 +941,35 @@ struct Solver {
         int cur = ia;
         for (int k = 0; k < titer; k++) {
             const int nxt = (k == 0) ? w1 : (cur == w1 ? w2 : w1);
-            for (int c0 = 0; c0 < nch; c0 += cb) {
-                const int ncb = std::min(cb, nch - c0);
-                const int z0 = c0 * nzp, zc = ncb * nzp;
-                forward_to_rays(d_f[cur].p + z0 * S, zc, d_g.p + (size_t)z0 * nt * n, nullptr,
-                                RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);
-                rays_to_images(zc);
-                TvArgs tb = ta;
-                tb.chunk0 = c0;
-                update(d_f[cur].p, A, d_f[nxt].p, tb, ncb, z0);
-            }
+            inner_step(d_f[cur].p, A, d_f[nxt].p, ta, k == 0);
             cur = nxt;
         }
-        prox(d_f[cur].p, A, ta);  // + reduce + rho rule
-        divchi(ta);               // reads the updated rho
+        prox(d_f[cur].p, A, ta, titer > 0);  // + reduce + rho rule
+        divchi(ta);                          // reads the updated rho
         ia = cur;
+    }
+
+    // One inner ADMM step (admm.cu:85-97) over all chunks: nxt = cur - 1/2 div(rho
+    // grad cur + chi) - 1/2 R^*(R cur - g), halo slabs from A.  With data_term the
+    // ray passes also leave the partial sums of ||R cur - g||^2, reduced into d_red[5].
+    void inner_step(const float *cur, const float *A, float *nxt, const TvArgs &ta, bool data_term) {
+        const size_t S = (size_t)M * n * n;
+        size_t roff = 0;
+        for (int c0 = 0; c0 < nch; c0 += cb) {
+            const int ncb = std::min(cb, nch - c0);
+            const int z0 = c0 * nzp, zc = ncb * nzp;
+            forward_to_rays(cur + z0 * S, zc, d_g.p + (size_t)z0 * nt * n, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ,
+                            data_term ? d_rpart.p + roff : nullptr);
+            roff += ray_blocks(zc);
+            rays_to_images(zc);
+            TvArgs tb = ta;
+            tb.chunk0 = c0;
+            update(cur, A, nxt, tb, ncb, z0);
+        }
+        if (data_term)
+            launch("reduce", [&] {
+                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_rpart.p, d_red.p + 5, (int)roff, 1);
+            });
     }
 
     void graph_iteration(int titer, float lam0, float lam1) {
@@ -1005,12 +1026,12 @@ struct Solver {
                     "history rows requested beyond the iterations run since reset");
         RTV_REQUIRE(iters_done - first <= HIST_CAP, "history rows no longer held (ring of 1024 iterations)");
         if (count == 0) return;
-        std::vector<double> ring((size_t)HIST_CAP * 5);
+        std::vector<double> ring((size_t)HIST_CAP * HIST_W);
         RTV_HIP(hipMemcpyAsync(ring.data(), d_hist.p, ring.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         RTV_HIP(hipStreamSynchronize(st));
         for (int q = 0; q < count; q++) {
             const size_t row = (size_t)((first + q) % HIST_CAP);
-            for (int c = 0; c < 5; c++) out[5 * (size_t)q + c] = ring[5 * row + c];
+            for (int c = 0; c < HIST_W; c++) out[HIST_W * (size_t)q + c] = ring[HIST_W * row + c];
         }
     }
 
@@ -1024,14 +1045,15 @@ struct Solver {
             const int blk = quiet ? std::min(HIST_CAP, niter - done) : 1;
             const long long first = iters_done;
             enqueue(blk, titer, lam0, lam1);
-            h.resize((size_t)5 * blk);
+            h.resize((size_t)HIST_W * blk);
             history(first, blk, h.data());
             collect();
             for (int q = 0; q < blk; q++) {
                 const int it = done + q;
                 if (hist)
-                    for (int c = 0; c < 4; c++) hist[4 * (size_t)it + c] = h[5 * (size_t)q + c];
-                if (!quiet) fprintf(stderr, "iter (%d/%d), rdiff ||f(k+1)-f(k)||:%f \n", it, niter, h[5 * (size_t)q + 3]);
+                    for (int c = 0; c < 4; c++) hist[4 * (size_t)it + c] = h[HIST_W * (size_t)q + c];
+                if (!quiet)
+                    fprintf(stderr, "iter (%d/%d), rdiff ||f(k+1)-f(k)||:%f \n", it, niter, h[HIST_W * (size_t)q + 3]);
             }
             done += blk;
         }
@@ -1041,6 +1063,74 @@ struct Solver {
         RTV_REQUIRE(fres, "null output pointer");
         RTV_HIP(hipMemcpyAsync(fres, d_f[ia].p, (size_t)nz * M * n * n * sizeof(float), hipMemcpyDeviceToHost, st));
         RTV_HIP(hipStreamSynchronize(st));
+    }
+
+    // ------------------------------------------------------------ state (checkpoint / restore)
+    // f: the previous iterate A [nz][M][n][n]; psi, mu [nch][nzp+1][M+1][n+1][n+1][4]
+    // (NULL: zero); rho > 0.  div(mu - rho psi) is formed for lambda1, the outer
+    // iteration counter and history restart at zero.  Replaces the zero start of
+    // rectv::run (rectv.cu:119-125) for callers that resume a saved state.
+    void set_state(const float *f, const float *psi, const float *mu, float rho, float lam1) {
+        RTV_REQUIRE(std::isfinite(rho) && rho > 0.f, "rho must be positive and finite");
+        RTV_REQUIRE(std::isfinite(lam1) && lam1 != 0.f, "lambda1 must be finite and nonzero");
+        const size_t V = (size_t)nz * M * n * n;
+        if (f) RTV_HIP(hipMemcpyAsync(d_f[ia].p, f, V * sizeof(float), hipMemcpyHostToDevice, st));
+        else RTV_HIP(hipMemsetAsync(d_f[ia].p, 0, V * sizeof(float), st));
+        if (psi) RTV_HIP(hipMemcpyAsync(d_psi.p, psi, d_psi.n * sizeof(float4), hipMemcpyHostToDevice, st));
+        else RTV_HIP(hipMemsetAsync(d_psi.p, 0, d_psi.n * sizeof(float4), st));
+        if (mu) RTV_HIP(hipMemcpyAsync(d_mu.p, mu, d_mu.n * sizeof(float4), hipMemcpyHostToDevice, st));
+        else RTV_HIP(hipMemsetAsync(d_mu.p, 0, d_mu.n * sizeof(float4), st));
+        static const int zero = 0;
+        RTV_HIP(hipMemcpyAsync(d_rho.p, &rho, sizeof(float), hipMemcpyHostToDevice, st));
+        RTV_HIP(hipMemcpyAsync(d_iter.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+        iters_done = 0;
+        divchi(tv_args(lam1, 0.f));
+        RTV_HIP(hipStreamSynchronize(st));
+        collect();
+    }
+
+    void get_state(float *f, float *psi, float *mu, float *dchi, float *rho) {
+        const size_t V = (size_t)nz * M * n * n;
+        if (f) RTV_HIP(hipMemcpyAsync(f, d_f[ia].p, V * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (psi) RTV_HIP(hipMemcpyAsync(psi, d_psi.p, d_psi.n * sizeof(float4), hipMemcpyDeviceToHost, st));
+        if (mu) RTV_HIP(hipMemcpyAsync(mu, d_mu.p, d_mu.n * sizeof(float4), hipMemcpyDeviceToHost, st));
+        if (dchi) RTV_HIP(hipMemcpyAsync(dchi, d_dchi.p, V * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (rho) RTV_HIP(hipMemcpyAsync(rho, d_rho.p, sizeof(float), hipMemcpyDeviceToHost, st));
+        RTV_HIP(hipStreamSynchronize(st));
+    }
+
+    // One inner step of the hot path on a given iterate fcur (host), against the
+    // resident previous iterate, psi, mu, rho and data: fout = fcur - 1/2 div(rho grad
+    // fcur + mu - rho psi) - 1/2 R^*(R fcur - g) (admm.cu:85-97).
+    void apply_inner_step(const float *fcur, float *fout, float lam1) {
+        RTV_REQUIRE(inputs, "inputs not set");
+        RTV_REQUIRE(fcur && fout, "null array");
+        const size_t V = (size_t)nz * M * n * n;
+        const int w1 = (ia + 1) % 3, w2 = (ia + 2) % 3;
+        RTV_HIP(hipMemcpyAsync(d_f[w1].p, fcur, V * sizeof(float), hipMemcpyHostToDevice, st));
+        inner_step(d_f[w1].p, d_f[ia].p, d_f[w2].p, tv_args(lam1, 0.f), false);
+        RTV_HIP(hipMemcpyAsync(fout, d_f[w2].p, V * sizeof(float), hipMemcpyDeviceToHost, st));
+        RTV_HIP(hipStreamSynchronize(st));
+        collect();
+    }
+
+    // The final step of an outer iteration on a given iterate fcur (host): prox and
+    // dual update of the resident psi, mu (admm.cu:101-117), the norms, the rho rule
+    // (rectv.cu:224-232) and div(mu - rho psi) for the next iteration.  The resident
+    // previous iterate is kept.  row receives the history row (HIST_W doubles).
+    void apply_prox(const float *fcur, float lam0, float lam1, double *row) {
+        RTV_REQUIRE(fcur, "null array");
+        RTV_REQUIRE(std::isfinite(lam0) && std::isfinite(lam1) && lam1 != 0.f, "lambda1 must be finite and nonzero");
+        const size_t V = (size_t)nz * M * n * n;
+        const int w1 = (ia + 1) % 3;
+        RTV_HIP(hipMemcpyAsync(d_f[w1].p, fcur, V * sizeof(float), hipMemcpyHostToDevice, st));
+        const TvArgs ta = tv_args(lam1, lam0);
+        prox(d_f[w1].p, d_f[ia].p, ta, false);
+        divchi(ta);
+        iters_done++;
+        if (row) history(iters_done - 1, 1, row);
+        RTV_HIP(hipStreamSynchronize(st));
+        collect();
     }
 
     // ------------------------------------------------------------ operator entry points
@@ -1280,6 +1370,7 @@ struct Solver {
         d_q.release(t);
         d_h1.release(t);
         d_part.release(t);
+        d_rpart.release(t);
         d_red.release(t);
         d_geo.release(t);
         d_aitems.release(t);
@@ -1396,6 +1487,19 @@ int rtv_check_adjoints(rtv_solver *h, double *res, const float *g, const float *
         RTV_REQUIRE(res, "null result pointer");
         live(h)->check_adjoints(res, g, theta, phi, lambda1, center);
     });
+}
+
+int rtv_set_state(rtv_solver *h, const float *f, const float *psi, const float *mu, float rho, float lambda1) {
+    return guarded([&] { live(h)->set_state(f, psi, mu, rho, lambda1); });
+}
+int rtv_get_state(rtv_solver *h, float *f, float *psi, float *mu, float *dchi, float *rho) {
+    return guarded([&] { live(h)->get_state(f, psi, mu, dchi, rho); });
+}
+int rtv_apply_inner_step(rtv_solver *h, const float *fcur, float *fout, float lambda1) {
+    return guarded([&] { live(h)->apply_inner_step(fcur, fout, lambda1); });
+}
+int rtv_apply_prox(rtv_solver *h, const float *fcur, float lambda0, float lambda1, double *row) {
+    return guarded([&] { live(h)->apply_prox(fcur, lambda0, lambda1, row); });
 }
 
 int rtv_apply_radon(rtv_solver *h, const float *f, float *g) {

@@ -114,12 +114,61 @@ class Solver(rectv):
         _lib.check(_lib.lib().rtv_iterations(self._handle(), ctypes.byref(v)))
         return int(v.value)
 
+    HIST_COLUMNS = ("r", "s", "rho", "rdiff", "fnorm2", "data", "tv")
+
     def history(self, first, count):
-        """Rows [first, first+count) since reset as [count, 5]: r, s, rho used,
-        ||f(k+1)-f(k)||^2, ||f(k+1)||^2 (waits for the solver's stream)."""
-        out = np.zeros((max(count, 1), 5))
+        """Rows [first, first+count) since reset as [count, 7]: r, s, rho used,
+        ||f(k+1)-f(k)||^2, ||f(k+1)||^2, 1/2||R f(k) - g||^2 of the entering iterate
+        (NaN when titer = 0) and sum |grad f(k+1)| (waits for the solver's stream)."""
+        out = np.zeros((max(count, 1), len(self.HIST_COLUMNS)))
         _lib.check(_lib.lib().rtv_history(self._handle(), int(first), int(count), getp(out)))
         return out[:count]
+
+    @staticmethod
+    def objective(rows, lambda0):
+        """Objective J(f_k) = 1/2||R f_k - g||^2 + lambda0/2 sum|grad f_k| of the iterates
+        f_1 .. f_{count-1} from history rows 0 .. count-1 (lambda0 as given to
+        iterate, i.e. already multiplied by lambda1)."""
+        rows = np.asarray(rows)
+        return rows[1:, 5] + 0.5 * lambda0 * rows[:-1, 6]
+
+    # ---- state (checkpoint / restore) -------------------------------------------
+    def dual_shape(self):
+        return (self.nz // self.nzp, self.nzp + 1, self.m + 1, self.n + 1, self.n + 1, 4)
+
+    def set_state(self, f=None, psi=None, mu=None, rho=0.5, lambda1=1.0):
+        """Resume from a saved state (None: zero); see rtv_set_state."""
+        V = (self.nz, self.m, self.n, self.n)
+        f = None if f is None else _f32(f, V, "f")
+        psi = None if psi is None else _f32(psi, self.dual_shape(), "psi")
+        mu = None if mu is None else _f32(mu, self.dual_shape(), "mu")
+        p = lambda a: None if a is None else getp(a)
+        _lib.check(_lib.lib().rtv_set_state(self._handle(), p(f), p(psi), p(mu), float(rho), float(lambda1)))
+
+    def get_state(self):
+        """dict(f, psi, mu, dchi, rho) of the resident state."""
+        V = (self.nz, self.m, self.n, self.n)
+        out = dict(f=np.zeros(V, np.float32), psi=np.zeros(self.dual_shape(), np.float32),
+                   mu=np.zeros(self.dual_shape(), np.float32), dchi=np.zeros(V, np.float32))
+        rho = np.zeros(1, np.float32)
+        _lib.check(_lib.lib().rtv_get_state(self._handle(), getp(out["f"]), getp(out["psi"]), getp(out["mu"]),
+                                            getp(out["dchi"]), getp(rho)))
+        out["rho"] = float(rho[0])
+        return out
+
+    def apply_inner_step(self, fcur, lambda1):
+        V = (self.nz, self.m, self.n, self.n)
+        fcur = _f32(fcur, V, "fcur")
+        out = np.zeros(V, np.float32)
+        _lib.check(_lib.lib().rtv_apply_inner_step(self._handle(), getp(fcur), getp(out), float(lambda1)))
+        return out
+
+    def apply_prox(self, fcur, lambda0, lambda1):
+        """Returns the history row (7 values) of the closing phase applied to fcur."""
+        fcur = _f32(fcur, (self.nz, self.m, self.n, self.n), "fcur")
+        row = np.zeros(len(self.HIST_COLUMNS))
+        _lib.check(_lib.lib().rtv_apply_prox(self._handle(), getp(fcur), float(lambda0), float(lambda1), getp(row)))
+        return row
 
     def set_graph(self, on=True):
         """Capture each outer iteration once as a HIP graph and replay it."""

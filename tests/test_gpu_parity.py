@@ -61,13 +61,30 @@ def _case(name):
         n, nt, m, nz = 2048, 6, 2, 1
         th = B.angles(3, 2)
         phi, center = B.takephi(nt, m), n / 2
+    elif name == "n480_radix5":           # n = 480 = 8*4*3*5, 2n = 960 = 8*8*3*5
+        n, nt, m, nz = 480, 10, 3, 1
+        th = B.angles(5, 2)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
+    elif name == "n504_radix7":           # the reference test's n: 504 = 8*3*3*7, 2n = 1008 = 8*2*3*3*7
+        n, nt, m, nz = 504, 12, 4, 1
+        th = np.linspace(0, 8 * np.pi, nt).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2 - 1.5
+    elif name == "n40_radix5_small":      # n = 40 = 8*5, 2n = 80 = 16*5
+        n, nt, m, nz = 40, 14, 5, 2
+        th = B.angles(7, 2)
+        phi, center = B.takephi(nt, m), n / 2 + 0.25
+    elif name == "n56_radix7_small":      # n = 56 = 8*7, 2n = 112 = 16*7
+        n, nt, m, nz = 56, 15, 3, 2
+        th = np.sort(rng.uniform(0, 3 * np.pi, nt)).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2
     else:
         raise KeyError(name)
     return n, nt, m, nz, th, phi, center, rng
 
 
 CASES = ["n64_rand", "n72_mixed_radix_m3", "n384_mixed_radix", "cfg_small", "n64_m20_two_blocks",
-         "n1024_column_rounds", "n2048_column_rounds"]
+         "n1024_column_rounds", "n2048_column_rounds", "n480_radix5", "n504_radix7", "n40_radix5_small",
+         "n56_radix7_small"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -182,15 +199,17 @@ def test_check_adjoints_parity(gpu):
 
 
 def test_reference_adjoint_test_full_size(gpu):
-    """tests/test_adjoint.py:14-57 of the reference at its own size."""
-    from rectv_gpu_amd import Solver, takephi
+    """tests/test_adjoint.py:14-57 of the reference at its own size, through the
+    reference's import name (import rectv_gpu)."""
+    import rectv_gpu
+    from rectv_gpu_amd import takephi
     n, ntheta, m, nz, nzp, ngpus, lambda1 = 504, 2400, 16, 1, 1, 1, 8
     say("case reference adjoint test n=504 ntheta=2400 m=16")
     phi = takephi(ntheta, m)
     np.random.seed(0)
     data = np.random.random([nz, ntheta, n]).astype('float32')
     theta = np.linspace(0, 8 * np.pi, ntheta).astype('float32')
-    with Solver(n, ntheta, m, nz, nzp, ngpus) as slv:
+    with rectv_gpu.Solver(n, ntheta, m, nz, nzp, ngpus) as slv:
         res = slv.adjoint_tests(data, theta, phi, lambda1)
     say("  res", res)
     np.testing.assert_allclose(res[0], res[1], rtol=1e-1)
@@ -211,7 +230,8 @@ def _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed, basis="fourier", lam
     with Solver(n, nt, m, nz, nzp, 1) as s:
         s.set_inputs(data, th, phi)
         s.reset()
-        hist = s.iterate(niter, titer, lam0 * lam1, lam1, history=True)
+        s.iterate(niter, titer, lam0 * lam1, lam1)
+        hist = s.history(0, niter)
         out = s.result()
     h = []
     ref = O.run(data, th, phi, n, nt, m, nz, nzp, n / 2, lam0 * lam1, lam1, niter, titer, history=h)
@@ -225,13 +245,23 @@ def _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed, basis="fourier", lam
     (32, 4, 2, 2, 16, 19, 3, 2),     # m = 19: two MFMA column blocks, odd m
 ])
 def test_run_parity_small(gpu, n, nz, nzp, nrot, ntr, m, niter, titer):
+    """Result, rho sequence, r, s, ||f(k+1)-f(k)||^2, ||f(k+1)||^2 and both objective
+    terms per outer iteration against the FP64 oracle (trajectory compared up to the
+    first near-tie of the rho rule, which is reported)."""
+    import oracle_steps as OS
     say("case run", n, nz, nzp, nrot * ntr, m, niter, titer)
     out, ref, hist, h = _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed=n + nz)
     e = rel(out, ref)
     say("  rel L2", e, "rho gpu", list(hist[:, 2]), "rho ref", [x["rho"] for x in h])
-    assert e < RUN_TOL
-    assert list(hist[:, 2]) == [x["rho"] for x in h]
-    np.testing.assert_allclose(hist[:, 3], [x["rdiff"] for x in h], rtol=1e-3)
+    upto = OS.compare_trajectory(hist, h, rtol=RUN_TOL, say=say)
+    if upto == len(h):
+        assert e < RUN_TOL
+    lam0 = 1e-3 * 4.0
+    J = [x["data"] for x in h[1:]]
+    J = np.array(J) + 0.5 * lam0 * np.array([x["tv"] for x in h[:-1]])
+    from rectv_gpu_amd import Solver
+    say("  objective gpu", Solver.objective(hist, lam0), "oracle", J)
+    np.testing.assert_allclose(Solver.objective(hist, lam0), J, rtol=RUN_TOL)
 
 
 def test_run_in_slice_batches(gpu, monkeypatch):
@@ -245,6 +275,9 @@ def test_run_in_slice_batches(gpu, monkeypatch):
     e, e_same = rel(out, ref), rel(out, ref_all[0])
     say("  rel L2 vs oracle", e, "vs one batch", e_same)
     assert e < RUN_TOL and e_same < 1e-6
+    # per-batch partial sums of the data term land at their own offsets
+    np.testing.assert_allclose(hist[:, 5], ref_all[2][:, 5], rtol=1e-6)
+    np.testing.assert_allclose(hist[:, 5], [x["data"] for x in h], rtol=RUN_TOL)
 
 
 def test_run_parity_small_config(gpu):

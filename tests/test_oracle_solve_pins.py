@@ -97,7 +97,8 @@ def pin_chunk_order(O):
     a = O.run(*args, history=h0)
     b = O.run(*args, history=h1, order=[2, 1, 0])
     c = O.run(*args, history=h2, order=[1, 2, 0])
-    assert _rel(b, a) < 1e-12 and _rel(c, a) < 1e-12
+    d = O.run(*args, threads=3)
+    assert _rel(b, a) < 1e-12 and _rel(c, a) < 1e-12 and _rel(d, a) < 1e-12
     for x, y in zip(h0, h1):
         assert x["rho"] == y["rho"] and abs(x["r"] - y["r"]) <= 1e-12 * abs(x["r"])
 
