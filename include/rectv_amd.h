@@ -94,6 +94,21 @@ int rtv_apply_inner_step(rtv_solver *h, const float *fcur, float *fout, float la
  * rtv_history).  The resident previous iterate is unchanged. */
 int rtv_apply_prox(rtv_solver *h, const float *fcur, float lambda0, float lambda1, double *row);
 
+/* rtv_apply_prox on caller-owned DEVICE arrays, allocating nothing: no reference
+ * counterpart; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as rtv_apply_prox.  fcur, fprev [nz][m][n][n]; psi, mu
+ * [nz/nzp][nzp+1][m+1][n+1][n+1][4] (16-byte aligned, updated in place); dchi
+ * [nz][m][n][n] out; rho: 1 float in/out; row: 7 doubles out (as rtv_history, data
+ * term NaN).  Work is enqueued on `stream` (a hipStream_t; NULL = null stream) and
+ * the call returns without waiting.  Its scratch (partial sums) is a fixed 8 MiB
+ * array per device, held until the process ends: sizes needing more,
+ * 40 * ceil((n+1)/16)(m+1)(nzp+1)(nz/nzp) + 72 bytes, are rejected (RTV_ERR_ARG).
+ * Every call on one GPU shares that array, so calls that overlap in time on one
+ * GPU (other streams, other threads) race and must be serialised by the caller. */
+int rtv_prox_noalloc(const float *fcur, const float *fprev, float *psi, float *mu, float *dchi, float *rho,
+                     double *row, int n, int m, int nz, int nzp, float lambda0, float lambda1, void *stream);
+
 /* ---- operator entry points (parity tests).  Each applies one reference operator to
  * all nz slices, with inputs/outputs in host memory; rtv_set_inputs must have
  * been called (it supplies theta, phi and center). */

@@ -33,6 +33,7 @@ SIGNATURES = {
     "rtv_get_state": (_I, [_P, _P, _P, _P, _P, _P]),
     "rtv_apply_inner_step": (_I, [_P, _P, _P, _F]),
     "rtv_apply_prox": (_I, [_P, _P, _F, _F, _P]),
+    "rtv_prox_noalloc": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _F, _P]),
     "rtv_apply_radon": (_I, [_P, _P, _P]),
     "rtv_apply_radon_adj": (_I, [_P, _P, _P]),
     "rtv_apply_gradient": (_I, [_P, _P, _I, _F, _P]),

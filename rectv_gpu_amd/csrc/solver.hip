@@ -5,6 +5,8 @@
 #include "../../include/rectv_amd.h"
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1399,6 +1401,81 @@ struct Solver {
 
 }  // namespace rtv
 
+// ------------------------------------------------------- allocation-free closing phase
+// Scratch of rtv_prox_noalloc: the per-workgroup partial sums of k_prox, the
+// reduced sums and the history-row counter of k_rho_rule.  Each device holds its
+// own copy for the life of the process; host code uses the address returned by
+// hipGetSymbolAddress for the current device, never the array's name.
+constexpr size_t NOALLOC_DOUBLES = (size_t)1 << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+static constexpr int NOALLOC_MAX_DEV = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAX_DEV];
+
+// Device address of `scratch` on the current device (looked up once per device).
+static double *scratch_on_current_device() {
+    int dev = 0;
+    RTV_HIP(hipGetDevice(&dev));
+    RTV_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEV, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        RTV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        RTV_REQUIRE(p != nullptr, "no device address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// rtv_apply_prox on caller-owned device arrays, with the scratch above.
+static void prox_noalloc(const float *fcur, const float *fprev, float *psi, float *mu, float *dchi, float *rho,
+                         double *row, int n, int m, int nz, int nzp, float lam0, float lam1, hipStream_t stream) {
+    using namespace rtv;
+    RTV_REQUIRE(fcur && fprev && psi && mu && dchi && rho && row, "null device pointer");
+    RTV_REQUIRE(n > 0 && m > 0 && nz > 0 && nzp > 0 && nz % nzp == 0, "invalid sizes (nz must be a multiple of nzp)");
+    RTV_REQUIRE(m <= 32, "at most 32 basis functions are supported");
+    RTV_REQUIRE(((uintptr_t)psi % 16) == 0 && ((uintptr_t)mu % 16) == 0, "psi and mu must be 16-byte aligned");
+    RTV_REQUIRE(std::isfinite(lam0) && std::isfinite(lam1) && lam1 != 0.f, "lambda1 must be finite and nonzero");
+    const int nch = nz / nzp;
+    const size_t blocks = (size_t)((n + 1 + PX_ROWS - 1) / PX_ROWS) * (m + 1) * (nzp + 1) * nch;
+    const size_t red_off = ((PROX_W * blocks + 31) / 32) * 32;  // 256-byte aligned
+    const size_t need = red_off + 8 + 1;                         // reduced sums, row counter
+    if (need > NOALLOC_DOUBLES || blocks > (size_t)INT_MAX)
+        throw Error(RTV_ERR_ARG, "rtv_prox_noalloc: sizes need " + std::to_string(need * 8) +
+                                     " bytes of scratch, more than its " + std::to_string(NOALLOC_DOUBLES * 8) +
+                                     " (use rtv_apply_prox)");
+    double *sc = scratch_on_current_device();
+    double *part = sc, *red = sc + red_off;
+    int *counter = reinterpret_cast<int *>(red + 8);
+    TvArgs a;
+    a.n = n;
+    a.M = m;
+    a.nzp = nzp;
+    a.nchunk = nch;
+    a.chunk0 = 0;
+    const double c2 = std::cbrt(2.0);
+    a.gs = (float)(1.0 / (c2 * lam1));
+    a.gt = (float)(1.0 / c2);
+    a.ks = (float)(1.0 / (2.0 * c2 * lam1));
+    a.kt = (float)(1.0 / (2.0 * c2));
+    a.rho = rho;
+    a.lam0 = lam0;
+    float4 *p4 = reinterpret_cast<float4 *>(psi), *m4 = reinterpret_cast<float4 *>(mu);
+    hipLaunchKernelGGL(k_prox, dim3((unsigned)blocks), dim3(64 * PX_WAVES), 0, stream, fcur, fprev, p4, m4, a, part);
+    RTV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, stream, part, red, (int)blocks, PROX_W);
+    RTV_HIP(hipGetLastError());
+    // one history row (capacity 1: the counter's value does not matter)
+    hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, stream, red, rho, row, counter, 1, 0);
+    RTV_HIP(hipGetLastError());
+    const dim3 grid(Solver::tv_blocks(n, nch)), thr(TV_THREADS);
+    if (m <= 4) hipLaunchKernelGGL(k_divchi<4>, grid, thr, 0, stream, p4, m4, dchi, a);
+    else if (m <= 8) hipLaunchKernelGGL(k_divchi<8>, grid, thr, 0, stream, p4, m4, dchi, a);
+    else if (m <= 16) hipLaunchKernelGGL(k_divchi<16>, grid, thr, 0, stream, p4, m4, dchi, a);
+    else hipLaunchKernelGGL(k_divchi<32>, grid, thr, 0, stream, p4, m4, dchi, a);
+    RTV_HIP(hipGetLastError());
+}
+
 // ====================================================================== C ABI
 struct rtv_solver {
     rtv::Solver *s;
@@ -1500,6 +1577,14 @@ int rtv_apply_inner_step(rtv_solver *h, const float *fcur, float *fout, float la
 }
 int rtv_apply_prox(rtv_solver *h, const float *fcur, float lambda0, float lambda1, double *row) {
     return guarded([&] { live(h)->apply_prox(fcur, lambda0, lambda1, row); });
+}
+
+int rtv_prox_noalloc(const float *fcur, const float *fprev, float *psi, float *mu, float *dchi, float *rho,
+                     double *row, int n, int m, int nz, int nzp, float lambda0, float lambda1, void *stream) {
+    return guarded([&] {
+        prox_noalloc(fcur, fprev, psi, mu, dchi, rho, row, n, m, nz, nzp, lambda0, lambda1,
+                     static_cast<hipStream_t>(stream));
+    });
 }
 
 int rtv_apply_radon(rtv_solver *h, const float *f, float *g) {
