@@ -419,7 +419,7 @@ def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer, stats=No
 
 
 def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter, titer,
-        dtype=np.float64, log=None, history=None, order=None, state=None, threads=1):
+        dtype=np.float64, log=None, history=None, order=None, state=None, threads=1, tol=None):
     """rectv::run, rectv.cu:102-252 (lambda0 arrives already multiplied by lambda1).
 
     Chunks of nzp slices are processed against the previous outer iterate f (halo
@@ -444,6 +444,9 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
     f, psi, mu and rho.
     ``threads`` > 1 processes that many chunks at a time on host threads (chunks
     read f and write disjoint parts of fn, psi and mu, so the result is the same).
+    ``tol``: stop after the first outer iteration whose relative change
+    sqrt(||fn - f||^2 / ||fn||^2) is below tol (the early stop of the serving loop;
+    the reference always runs niter iterations).
     Returns f [nz, m, n, n].
     """
     if nz % nzp != 0:
@@ -504,6 +507,9 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
             history.append(dict(iter=it, r=r, s=s, rho=rho_used, rho_next=rho, rdiff=norm,
                                 fnorm=float(np.sum(fn.astype(np.float64) ** 2)), data=data_t, tv=tv_t))
         f, fn = fn, f
+        fn2 = float(np.sum(f.astype(np.float64) ** 2))
+        if tol is not None and fn2 > 0 and np.sqrt(norm / fn2) < tol:
+            break
     if pool is not None:
         pool.shutdown()
     if state is not None:
