@@ -82,6 +82,23 @@ int rtv_set_state(rtv_solver *h, const float *f, const float *psi, const float *
 /* Copies the state out (any pointer may be NULL; rho: 1 float). */
 int rtv_get_state(rtv_solver *h, float *f, float *psi, float *mu, float *dchi, float *rho);
 
+/* ---- filtered back-projection and support mask: the reference's unused USFFT
+ * extras applyfilter (src/include/kernels_radonusfft.cuh:173-189), the `filter`
+ * argument of radonusfft::adjR (src/cuda/radonusfft.cu:106) and circ
+ * (kernels_radonusfft.cuh:146-159). */
+/* r >= 0: keep pixels with 4x^2 + 4y^2 < 1 - r, x = (col - n/2)/n (circ), in the
+ * filtered back-projection and after every inner step (a support constraint);
+ * r < 0 (the default): no mask, the reference's iteration. */
+int rtv_set_support(rtv_solver *h, float r);
+/* f [nz][m][n][n] = in every frame pi/(4 sqrt(m)) Re adjR_w(g): the filtered
+ * back-projection of each slice of g [nz][ntheta][n] as a static object (geometry
+ * from rtv_set_inputs), masked by the support when set. */
+int rtv_apply_fbp(rtv_solver *h, const float *g, float *f);
+/* Starting state from the filtered back-projection of the resident data (instead of
+ * f = 0, src/cuda/rectv.cu:119-125): f = FBP, psi = mu = 0, rho = 0.5; then
+ * rtv_iterate continues from it. */
+int rtv_fbp_start(rtv_solver *h);
+
 /* ---- single steps of the ADMM iteration on the resident state (parity tests of
  * the fused kernels; src/cuda/admm.cu:74-122 split at its two phases). */
 /* One inner step (admm.cu:85-97) of every chunk applied to fcur [nz][m][n][n]:

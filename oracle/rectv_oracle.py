@@ -192,14 +192,21 @@ class Usfft:
         g *= 1.0 / np.sqrt(n * self.ntheta)                         # mulr (:102)
         return g.astype(self.cdtype)
 
-    def adjR(self, g):
-        """Adjoint USFFT Radon, radonusfft.cu:106-135.  g [B, ntheta, n] -> [B, n, n]."""
+    def adjR(self, g, filter=False):
+        """Adjoint USFFT Radon, radonusfft.cu:106-135.  g [B, ntheta, n] -> [B, n, n].
+
+        filter=True multiplies the centred detector spectrum by the applyfilter
+        weights (kernels_radonusfft.cuh:173-189) right after the forward FFT and its
+        checkerboard: the `filter` argument of radonusfft::adjR (radonusfft.cu:106,
+        radonusfft.cuh:37), which the reference's body leaves unused."""
         n, K, Ng = self.n, self.K, self.Ng
         B = g.shape[0]
         sg = sigma(np.arange(n))
         g = g * sg                                                  # fftshift 1D (:115)
         g = sfft.fft(g, axis=-1, workers=_WORKERS)                  # CUFFT_FORWARD (:117)
         g *= sg                                                     # fftshift 1D (:118)
+        if filter:
+            g = g * applyfilter_weights(n)[None, None, :]
         g *= self.shiftadj[None, None, :]                            # shift (:121)
         g = g / n                                                   # gather ADJ g0 = g/n (:102-103)
         fde = (self.WT @ g.reshape(B, self.ntheta * n).T).T.reshape(B, Ng, Ng)  # scatter (:126-131)
@@ -238,6 +245,27 @@ class Usfft:
         flat = fde.reshape(B, -1)
         np.add.at(flat, (slice(None), src[yy] * self.Ng + src[xx]), flat[:, yy * self.Ng + xx])
         return flat.reshape(fde.shape)
+
+
+def applyfilter_weights(n):
+    """FBP filter of kernels_radonusfft.cuh:173-189 on the centred detector spectrum:
+    rho = (s - n/2)/n (integer n/2), w = 0 at rho == 0, else |rho| n 4 sin(rho)/rho."""
+    rho = (np.arange(n) - n // 2) / float(n)
+    safe = np.where(rho == 0, 1.0, rho)
+    return np.where(rho == 0, 0.0, np.abs(rho) * n * 4 * np.sin(safe) / safe)
+
+
+def circ_mask(n, r):
+    """Support mask of kernels_radonusfft.cuh:146-159, evaluated in float32 as there:
+    x = (tx - n/2)/float(n), y likewise, kept where 4x^2 + 4y^2 < 1 - r."""
+    t = ((np.arange(n) - n // 2).astype(np.float32) / np.float32(n)).astype(np.float32)
+    q = np.float32(4) * t[None, :] * t[None, :] + np.float32(4) * t[:, None] * t[:, None]
+    return (q < np.float32(1) - np.float32(r)).astype(np.float64)
+
+
+def circ(f, r):
+    """f[..., y, x] *= circ mask (kernels_radonusfft.cuh:146-159)."""
+    return f * circ_mask(f.shape[-1], r)
 
 
 # ---------------------------------------------------------------------------
@@ -326,6 +354,19 @@ class Operators:
         """fn += tau * div(h2) (operators.cu:68-71)."""
         return div(fn, h2, tau, self.lambda1)
 
+    def fbp(self, g, r=None):
+        """Filtered back-projection of every slice as a static object, the same image in
+        each of the m frames: u = pi/(4 sqrt(m)) Re adjR(g, filter=True).  For a basis
+        holding the constant function with the others orthogonal to constants at the
+        frame times, R_apr of a static object u is sqrt(m) R u (operators.cu:6-32), and
+        pi/4 R^*_w inverts R for the applyfilter weights w ~ 4 n |rho| with R, R^*
+        scaled by 1/sqrt(n ntheta) (radonusfft.cu:102, :132).  r: optional support mask
+        (circ).  Returns [nz, m, n, n]."""
+        u = np.pi / (4 * np.sqrt(self.m)) * self.rad.adjR(np.asarray(g).astype(self.cdtype), filter=True).real
+        if r is not None:
+            u = circ(u, r)
+        return np.repeat(u[:, None], self.m, axis=1).astype(self.dtype)
+
 
 def extendf(fext, flgl, flgr):
     """Clamped/halo extension E [nzp+2, m+2, n+2, n+2], kernels_operators.cuh:4-20."""
@@ -382,7 +423,7 @@ def solve_reg(h2, mu, lam, rho):
     return out.astype(h2.dtype)
 
 
-def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer, stats=None):
+def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer, stats=None, support=None):
     """One chunk's ADMM step, admm.cu:74-122.
 
     fext [nzp+2, m, n, n] is the working copy fm with halo slabs; it is updated in
@@ -394,6 +435,8 @@ def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer, stats=No
     enters (the first inner step's residual h1; computed directly when titer = 0)
     and ``tv`` = sum over dual positions of the 4-vector norm of the gradient h2 of
     the updated chunk (the h2 of admm.cu:101, halos from the previous iterate).
+    ``support`` (optional r): after each inner step the frames are multiplied by the
+    circ mask of radius r (a support constraint; the reference has none).
     """
     inner = fext[1:-1]
     hs = ops.gradient(fext, flgl, flgr)                         # h2stored = grad fm (:79)
@@ -406,6 +449,8 @@ def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer, stats=No
             stats["data"] = 0.5 * float(np.sum(h1.astype(np.float64) ** 2))
         ops.divergent(inner, h2, -0.5)                          # (:95)
         ops.radonapradj(inner, h1, -0.5)                        # (:97)
+        if support is not None:
+            inner *= circ_mask(inner.shape[-1], support)
     if titer == 0 and stats is not None:
         stats["data"] = 0.5 * float(np.sum((ops.radonapr(inner, 1.0) - g).astype(np.float64) ** 2))
     h2 = ops.gradient(fext, flgl, flgr)                         # (:101)
@@ -419,7 +464,8 @@ def solver_admm(ops, fext, g, psi, mu, lambda0, rho, flgl, flgr, titer, stats=No
 
 
 def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter, titer,
-        dtype=np.float64, log=None, history=None, order=None, state=None, threads=1, tol=None):
+        dtype=np.float64, log=None, history=None, order=None, state=None, threads=1, tol=None,
+        support=None):
     """rectv::run, rectv.cu:102-252 (lambda0 arrives already multiplied by lambda1).
 
     Chunks of nzp slices are processed against the previous outer iterate f (halo
@@ -447,6 +493,7 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
     ``tol``: stop after the first outer iteration whose relative change
     sqrt(||fn - f||^2 / ||fn||^2) is below tol (the early stop of the serving loop;
     the reference always runs niter iterations).
+    ``support``: the circ-mask radius parameter of solver_admm (None: no mask).
     Returns f [nz, m, n, n].
     """
     if nz % nzp != 0:
@@ -483,7 +530,7 @@ def run(data, theta, phi, n, ntheta, m, nz, nzp, center, lambda0, lambda1, niter
             if flgr:
                 fext[-1] = f[lo + nzp]
             nd[iz] = solver_admm(ops, fext, g[lo:lo + nzp], psi[iz], mu[iz],
-                                 lambda0, rho, flgl, flgr, titer, stats=sts[iz])
+                                 lambda0, rho, flgl, flgr, titer, stats=sts[iz], support=support)
             fn[lo:lo + nzp] = fext[1:-1]
 
         if pool is None:

@@ -34,6 +34,9 @@ SIGNATURES = {
     "rtv_apply_inner_step": (_I, [_P, _P, _P, _F]),
     "rtv_apply_prox": (_I, [_P, _P, _F, _F, _P]),
     "rtv_prox_noalloc": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _F, _P]),
+    "rtv_set_support": (_I, [_P, _F]),
+    "rtv_apply_fbp": (_I, [_P, _P, _P]),
+    "rtv_fbp_start": (_I, [_P]),
     "rtv_apply_radon": (_I, [_P, _P, _P]),
     "rtv_apply_radon_adj": (_I, [_P, _P, _P]),
     "rtv_apply_gradient": (_I, [_P, _P, _I, _F, _P]),

@@ -988,6 +988,31 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
     }
 }
 
+// Pixel (x, y) lies in the support mask (circ, kernels_radonusfft.cuh:146-159): the
+// host evaluates 4 xf^2 + 4 yf^2 < 1 - r in float per pixel and stores each row's
+// kept columns as one range, so the kernels compare integers only.
+__device__ __forceinline__ bool in_support(const int2 *supp, int x, int y) {
+    if (!supp) return true;
+    const int2 r = supp[y];
+    return x >= r.x && x < r.y;
+}
+
+// Filtered back-projection as the starting iterate: every frame i of slice z gets
+// kappa * Re v[z][0] (the adjoint of the filtered rays through a basis that selects
+// one static frame), masked by the support when one is set.
+__global__ void k_fbp_frames(float *__restrict__ f, const float2 *__restrict__ v, int n, int M, float kappa,
+                             const int2 *__restrict__ supp) {
+    const size_t plane = (size_t)n * n;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (p >= plane) return;
+    const int MP = (M + 1) / 2;
+    const int y = (int)(p / n), x = (int)(p - (size_t)y * n);
+    const float val = in_support(supp, x, y) ? kappa * v[(size_t)z * MP * plane + p].x : 0.f;
+    float *fz = f + (size_t)z * M * plane + p;
+    for (int i = 0; i < M; i++) fz[(size_t)i * plane] = val;
+}
+
 // f[z][i] (+)= tau * frame i of the adjoint pairs v (re of pair i/2 for even i,
 // im for odd i): recphi + addreal, kernels_operators.cuh:118-133 and :75-84.
 // Used by the standalone R^* entry point.
@@ -1016,7 +1041,9 @@ struct TvArgs {
     float gs, gt, ks, kt;
     const float *rho;  // device-resident penalty (updated by k_rho_rule between outer iterations)
     float lam0;
+    const int2 *supp = nullptr;  // support mask (kept x range [lo, hi) per row y), null: none
 };
+
 
 // TV kernels: workgroups of 64 x TV_ROWS threads over (x, y) tiles of one chunk.
 // The 1D grid (x tiles fastest, then y tiles, then chunks) is remapped so that
@@ -1116,6 +1143,7 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
     const unsigned oyp = 4u * ((y < n - 1 ? y + 1 : y) * n + x), oym = 4u * ((y > 0 ? y - 1 : y) * n + x);
     const float r = *a.rho;
     const float rgs = r * a.gs, rgt = r * a.gt;
+    const float keep = in_support(a.supp, x, y) ? 1.f : 0.f;  // projection onto the support
     float fz[MM], fzp[MM], hzc[MM];
     // own column at slice 0 and the z-gradient values on plane 0
     {
@@ -1162,7 +1190,7 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
                 const float hz0 = rgs * (fzp[i] - f0);
                 const float dv = a.ks * (hxm - hx0) + a.ks * (hym - hy0) + a.kt * (htc - ht0) + a.ks * (hzc[i] - hz0) +
                                  buf_ld_nt(rd, oc, so);
-                buf_st_nt(f0 - 0.5f * dv - 0.5f * rpi, ro, oc, so);
+                buf_st_nt((f0 - 0.5f * dv - 0.5f * rpi) * keep, ro, oc, so);
                 htc = ht0;
                 hzc[i] = hz0;
             }

@@ -78,9 +78,18 @@ struct HostGeom {
     std::vector<int4> achunk;  // per adjoint chunk: {tile, first record, count, 0}
     std::vector<int2> runs;    // per row of G: circular run {lo, len} of the columns inside some window
     std::vector<float2> pre_f, post_a, cf;
+    std::vector<float2> post_aw;    // post_a times the FBP filter (filtered back-projection)
+    std::vector<float2> cf_static;  // pair coefficients of a basis selecting one static frame
     std::vector<float> post_f, pre_a, deap;
     float beta = 0.f;
 };
+
+// FBP filter of kernels_radonusfft.cuh:173-189 at detector frequency index s of the
+// centred spectrum: rho = (s - n/2)/n, w = 0 at rho == 0, else |rho| n 4 sin(rho)/rho.
+static double fbp_filter(int s, int n) {
+    const double rho = (double)(s - n / 2) / (double)n;
+    return rho == 0.0 ? 0.0 : std::fabs(rho) * n * 4 * std::sin(rho) / rho;
+}
 
 // USFFT constants, radonusfft.cu:10-12: mu = -log(eps)/(2 n^2) held in float.
 static float usfft_mu(int n) {
@@ -359,6 +368,7 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
     hg.post_f.resize(n);
     hg.pre_a.resize(n);
     hg.post_a.resize(n);
+    hg.post_aw.resize(n);
     for (int s = 0; s < n; s++) {
         const double sg = 1 - 2 * ((s + 1) % 2);
         const double af = 2 * PI_REF * cfw * (s - n / 2.0) / n;
@@ -368,6 +378,13 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         hg.pre_a[s] = (float)sg;
         hg.post_a[s] = s == 0 ? make_float2(0.f, 0.f)
                               : make_float2((float)(sg * cst * std::cos(aa)), (float)(sg * cst * std::sin(aa)));
+        // the filter acts on the centred spectrum right after the forward FFT, where
+        // the `filter` argument of radonusfft::adjR (radonusfft.cu:106) would apply it;
+        // the ray tables' basis scale ntheta/sqrt(M) is divided out (the static basis
+        // below carries no such scale)
+        const double wf = fbp_filter(s, n) * std::sqrt((double)M) / nt;
+        hg.post_aw[s] = s == 0 ? make_float2(0.f, 0.f)
+                               : make_float2((float)(sg * cst * wf * std::cos(aa)), (float)(sg * cst * wf * std::sin(aa)));
     }
     // deapodisation 1/(2n exp(-mu (x - n/2)^2)) per axis (divphi, kernels_radonusfft.cuh:11-22)
     hg.deap.resize(n);
@@ -386,6 +403,13 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             pr[(size_t)j * nt + k] = phi[(size_t)j * 2 * nt + 2 * k];
             pim[(size_t)j * nt + k] = phi[(size_t)j * 2 * nt + 2 * k + 1];
         }
+    // a basis selecting one static frame: D[k][0] = 1, D[k][i > 0] = 0, i.e. pair 0 with
+    // alpha = beta = 1/2 (frame 0 of the adjoint frame pairs is Re R^* g)
+    hg.cf_static.assign((size_t)nt * 2 * MP, make_float2(0.f, 0.f));
+    for (int k = 0; k < nt; k++) {
+        hg.cf_static[((size_t)k * 2 + 0) * MP] = make_float2(0.5f, 0.f);
+        hg.cf_static[((size_t)k * 2 + 1) * MP] = make_float2(0.5f, 0.f);
+    }
     for (int k = 0; k < nt; k++) {
         std::vector<double> dr(2 * MP, 0.0), di(2 * MP, 0.0);
         for (int i = 0; i < M; i++) {
@@ -491,6 +515,9 @@ struct Solver {
     DBuf<int4> d_achunk;
     DBuf<float4> d_wtab;  // scatter weight rows per adjoint chunk (8 float4 per slot)
     DBuf<float2> d_cf, d_pre_f, d_post_a;
+    DBuf<float2> d_post_aw, d_cf_static;  // filtered back-projection tables
+    DBuf<int2> d_supp;                    // support mask: kept x range per row
+    float support_r = -1.f;               // < 0: no support mask
     DBuf<float> d_post_f, d_pre_a, d_deap;
     int nfm_entries = 0, na_tiles = 0;
     float beta = 0.f;
@@ -723,6 +750,8 @@ struct Solver {
         upload(d_post_f, hg.post_f, bytes, st);
         upload(d_pre_a, hg.pre_a, bytes, st);
         upload(d_post_a, hg.post_a, bytes, st);
+        upload(d_post_aw, hg.post_aw, bytes, st);
+        upload(d_cf_static, hg.cf_static, bytes, st);
         upload(d_deap, hg.deap, bytes, st);
         upload(d_runs, hg.runs, bytes, st);
         na_tiles = (int)hg.atiles.size();
@@ -806,41 +835,44 @@ struct Solver {
     }
 
     template <int ZW>
-    void scatter_t(int zc) {
+    void scatter_t(int zc, const float2 *cf) {
         const int blocks = na_tiles * ((zc + ZW - 1) / ZW);
         launch("scatter", [&] {
             // MP <= 16 (M <= 32): one or two 16-column blocks
             if (MP <= 8)
                 hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
-                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, d_wtab.p, n, nt, MP,
+                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, cf, d_wtab.p, n, nt, MP,
                                    na_tiles, zc);
             else
                 hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
-                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, d_cf.p, d_wtab.p, n, nt, MP,
+                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, cf, d_wtab.p, n, nt, MP,
                                    na_tiles, zc);
         });
     }
-    void scatter(int zc, int zw) {
-        if (zw == 2) scatter_t<2>(zc);
-        else if (zw == 8) scatter_t<8>(zc);
-        else scatter_t<4>(zc);
+    void scatter(int zc, int zw, const float2 *cf) {
+        if (zw == 2) scatter_t<2>(zc, cf);
+        else if (zw == 8) scatter_t<8>(zc, cf);
+        else scatter_t<4>(zc, cf);
     }
 
     int ray_blocks(int zc) const { return (zc * nt + T_ray - 1) / T_ray; }
-    void ray(int zc, const float *gdat, const float *h1in, float *h1out, int mode, double *partial) {
+    void ray(int zc, const float *gdat, const float *h1in, float *h1out, int mode, double *partial,
+             const float2 *post_a = nullptr) {
         const int nrays = zc * nt;
         const int blocks = ray_blocks(zc);
         const int thr = fft_threads(T_ray, n);
         const size_t lds = (size_t)T_ray * st_n * sizeof(float2);
+        const float2 *pa = post_a ? post_a : d_post_a.p;
         launch("ray", [&] {
             hipLaunchKernelGGL(pick_ray(n), dim3(blocks), dim3(thr), lds, st, d_gq.p, d_q.p, gdat, h1in, h1out, d_pre_f.p,
-                               d_post_f.p, d_pre_a.p, d_post_a.p, plan_n, d_tw.p, n, nrays, T_ray, st_n, mode, partial);
+                               d_post_f.p, d_pre_a.p, pa, plan_n, d_tw.p, n, nrays, T_ray, st_n, mode, partial);
         });
     }
 
-    // adjoint: d_q (q samples of zc slices) -> d_v images
-    void rays_to_images(int zc) {
-        scatter(zc, scatter_zw);
+    // adjoint: d_q (q samples of zc slices) -> d_v images (cf: pair coefficients,
+    // default the solver's basis)
+    void rays_to_images(int zc, const float2 *cf = nullptr) {
+        scatter(zc, scatter_zw, cf ? cf : d_cf.p);
         {
             const int blocks = zc * MP * (N / T_acols);
             const int thr = fft_threads(T_acols, N);
@@ -879,6 +911,7 @@ struct Solver {
         a.kt = (float)(1.0 / (2.0 * c2));
         a.rho = d_rho.p;
         a.lam0 = lam0;
+        a.supp = support_r >= 0.f ? d_supp.p : nullptr;
         return a;
     }
 
@@ -1135,6 +1168,83 @@ struct Solver {
         collect();
     }
 
+    // ------------------------------------------------------------ filtered back-projection, support
+    // Support mask circ(r) of kernels_radonusfft.cuh:146-159 (r < 0: none): kept pixels
+    // satisfy 4 xf^2 + 4 yf^2 < 1 - r with xf = (x - n/2)/float(n), evaluated in float
+    // without contractions, one contiguous column range per row.  Applied to the
+    // filtered back-projection and after every inner step (a support constraint).
+    void set_support(float r) {
+        RTV_REQUIRE(std::isfinite(r), "support radius parameter must be finite");
+        drop_graphs();
+        support_r = r;
+        if (r < 0.f) return;
+        std::vector<int2> rows(n, make_int2(0, 0));
+        const volatile float fn_ = (float)n, lim = 1.0f - r;
+        for (int y = 0; y < n; y++) {
+            int lo = -1, hi = -1;
+            for (int x = 0; x < n; x++) {
+                const volatile float xf = (float)(x - n / 2) / fn_, yf = (float)(y - n / 2) / fn_;
+                const volatile float ax = 4.0f * xf, ay = 4.0f * yf;
+                const volatile float qx = ax * xf, qy = ay * yf;
+                const volatile float q = qx + qy;
+                if (q < lim) {
+                    if (lo < 0) lo = x;
+                    RTV_REQUIRE(hi < 0 || hi == x, "internal: support row is not one interval");
+                    hi = x + 1;
+                }
+            }
+            if (lo >= 0) rows[y] = make_int2(lo, hi);
+        }
+        upload(d_supp, rows, bytes, st);
+        RTV_HIP(hipStreamSynchronize(st));
+    }
+
+    // f (device, [nz][M][n][n]) = every frame pi/(4 sqrt(M)) Re R^*_w g: the filtered
+    // back-projection of each slice as a static object (radonusfft::adjR with its
+    // filter argument set, the applyfilter weights of kernels_radonusfft.cuh:173-189),
+    // masked by the support when one is set.
+    void fbp_dev(const float *gdev, float *fdev) {
+        const size_t S = (size_t)M * n * n;
+        const float kappa = (float)(PI_EXACT / (4.0 * std::sqrt((double)M)));
+        const int2 *supp = support_r >= 0.f ? d_supp.p : nullptr;
+        for (int z0 = 0; z0 < nz; z0 += zb) {
+            const int zc = std::min(zb, nz - z0);
+            ray(zc, nullptr, gdev + (size_t)z0 * nt * n, nullptr, RAY_ADJ, nullptr, d_post_aw.p);
+            rays_to_images(zc, d_cf_static.p);
+            dim3 grid((unsigned)(((size_t)n * n + 255) / 256), zc);
+            launch("fbp", [&] {
+                hipLaunchKernelGGL(k_fbp_frames, grid, dim3(256), 0, st, fdev + z0 * S, d_v.p, n, M, kappa, supp);
+            });
+        }
+    }
+
+    void apply_fbp(const float *g, float *f) {
+        RTV_REQUIRE(inputs, "inputs not set");
+        RTV_REQUIRE(g && f, "null array");
+        const size_t S = (size_t)M * n * n;
+        DBuf<float> tf, th;
+        size_t tmp = 0;
+        tf.alloc((size_t)nz * S, tmp);
+        th.alloc((size_t)nz * nt * n, tmp);
+        RTV_HIP(hipMemcpyAsync(th.p, g, (size_t)nz * nt * n * sizeof(float), hipMemcpyHostToDevice, st));
+        fbp_dev(th.p, tf.p);
+        RTV_HIP(hipMemcpyAsync(f, tf.p, (size_t)nz * S * sizeof(float), hipMemcpyDeviceToHost, st));
+        RTV_HIP(hipStreamSynchronize(st));
+        collect();
+        tf.release(tmp);
+        th.release(tmp);
+    }
+
+    // Starting state from the filtered back-projection of the resident data: f = FBP,
+    // psi = mu = 0, rho = 0.5 (the reference starts from f = 0, rectv.cu:119-125).
+    void fbp_start() {
+        RTV_REQUIRE(inputs, "inputs not set");
+        reset();
+        fbp_dev(d_g.p, d_f[ia].p);
+        RTV_HIP(hipStreamSynchronize(st));
+        collect();
+    }
+
     // ------------------------------------------------------------ operator entry points
     void apply_radon(const float *f, float *g) {
         RTV_REQUIRE(inputs, "inputs not set");
@@ -1300,7 +1410,7 @@ struct Solver {
             }
             ray(zc, d_g.p, nullptr, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ, nullptr);  // q for the scatter timings
             for (int c : {2, 4, 8}) {
-                const float t = time_best([&] { scatter(zc, c); });
+                const float t = time_best([&] { scatter(zc, c, d_cf.p); });
                 if (t < bests) {
                     bests = t;
                     zw = c;
@@ -1392,6 +1502,9 @@ struct Solver {
         d_post_f.release(t);
         d_pre_a.release(t);
         d_deap.release(t);
+        d_post_aw.release(t);
+        d_cf_static.release(t);
+        d_supp.release(t);
         if (st) (void)hipStreamDestroy(st);
         st = nullptr;
         freed = true;
@@ -1585,6 +1698,16 @@ int rtv_prox_noalloc(const float *fcur, const float *fprev, float *psi, float *m
         prox_noalloc(fcur, fprev, psi, mu, dchi, rho, row, n, m, nz, nzp, lambda0, lambda1,
                      static_cast<hipStream_t>(stream));
     });
+}
+
+int rtv_set_support(rtv_solver *h, float r) {
+    return guarded([&] { live(h)->set_support(r); });
+}
+int rtv_apply_fbp(rtv_solver *h, const float *g, float *f) {
+    return guarded([&] { live(h)->apply_fbp(g, f); });
+}
+int rtv_fbp_start(rtv_solver *h) {
+    return guarded([&] { live(h)->fbp_start(); });
 }
 
 int rtv_apply_radon(rtv_solver *h, const float *f, float *g) {

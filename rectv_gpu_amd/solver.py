@@ -53,21 +53,62 @@ class Solver(rectv):
         phi = _f32(phi, (self.m, 2 * self.ntheta), "phi")
         return data, theta, phi
 
-    def recon(self, data, theta, phi, rot_center=None, lambda0=1e-3, lambda1=1, niter=1, titer=1):
+    def recon(self, data, theta, phi, rot_center=None, lambda0=1e-3, lambda1=1, niter=1, titer=1, *,
+              init=None, support=None):
         """Reconstruct (src/rectv_gpu/solver.py:45-84).
 
         data [nz, ntheta, n], theta [ntheta], phi [m, 2*ntheta] (re/im
         interleaved); returns float32 [nz, m, n, n].  As in the reference the
         TV weight passed to the native solver is lambda0*lambda1 and the
         rotation centre defaults to n/2.
+
+        Keyword-only extensions (defaults give the reference's iteration):
+        init="fbp" starts from the filtered back-projection of the data
+        (rtv_fbp_start) instead of zero; support=r >= 0 keeps every frame inside
+        the circ(r) mask after each inner step (rtv_set_support).
         """
         lambda0 = lambda0 * lambda1
         if rot_center is None:
             rot_center = self.n / 2
+        if init not in (None, "zero", "fbp"):
+            raise ValueError("init must be None, 'zero' or 'fbp'")
         data, theta, phi = self._inputs(data, theta, phi)
         rtv = np.zeros([self.nz, self.m, self.n, self.n], dtype='float32')
-        self.run(getp(rtv), getp(data), getp(theta), getp(phi), rot_center, lambda0, lambda1, niter, titer)
-        return rtv
+        if init in (None, "zero") and support is None:
+            self.run(getp(rtv), getp(data), getp(theta), getp(phi), rot_center, lambda0, lambda1, niter, titer)
+            return rtv
+        self.set_support(support)
+        try:
+            self.set_inputs(data, theta, phi, rot_center)
+            if init == "fbp":
+                _lib.check(_lib.lib().rtv_fbp_start(self._handle()))
+            else:
+                self.reset()
+            self.iterate(niter, titer, lambda0, lambda1)
+            return self.result()
+        finally:
+            self.set_support(None)
+
+    def set_support(self, r):
+        """Support mask circ(r) (kernels_radonusfft.cuh:146-159) on the filtered
+        back-projection and after every inner step; None or r < 0 removes it."""
+        _lib.check(_lib.lib().rtv_set_support(self._handle(), -1.0 if r is None else float(r)))
+
+    def fbp(self, data, theta=None, phi=None, rot_center=None):
+        """Filtered back-projection of data [nz, ntheta, n] as a static object, the same
+        image in each of the m frames: pi/(4 sqrt(m)) Re adjR_w(data) with the applyfilter
+        weights (kernels_radonusfft.cuh:173-189), masked by the support if set.  theta and
+        phi (with rot_center) set the geometry first when given."""
+        if theta is not None:
+            self.set_inputs(data, theta, phi, rot_center)
+        g = _f32(data, (self.nz, self.ntheta, self.n), "data")
+        f = np.zeros((self.nz, self.m, self.n, self.n), np.float32)
+        _lib.check(_lib.lib().rtv_apply_fbp(self._handle(), getp(g), getp(f)))
+        return f
+
+    def fbp_start(self):
+        """Start the resident state from the filtered back-projection of the resident data."""
+        _lib.check(_lib.lib().rtv_fbp_start(self._handle()))
 
     def adjoint_tests(self, data, theta, phi, lambda1, rot_center=None):
         """The six inner products of rectv::check_adjoints (src/rectv_gpu/solver.py:86-93)."""
