@@ -180,6 +180,27 @@ int rtv_batch_iterate(rtv_solver **hs, int count, const int *niter, const int *t
 /* Device bytes a solver of these sizes holds (upper bound), for admission control. */
 int rtv_footprint(int n, int ntheta, int m, int nz, int nzp, long long *bytes);
 
+/* ---- pre-processing of projection data before the solver (on the GPU, host arrays in
+ * and out; no solver handle).  The reference leaves these steps to its examples and
+ * to tomocupy: examples/tao/tao.py:34-43, 58-71 and examples/tao/rec_line.txt:1. */
+/* out [nz][ntheta][n] = -log((data - mean(dark)) / (1e-3 + mean(flat) - mean(dark)))
+ * (take_log = 0: the ratio itself) from data [ntheta][nz][n], dark [ndark][nz][n],
+ * flat [nflat][nz][n], frame means over the first axis (tao.py:34-43, including its
+ * swapaxes(0, 1) to sinograms). */
+int rtv_prep_darkflat(const float *data, const float *dark, const float *flat, int ntheta, int nz, int n,
+                      int ndark, int nflat, int take_log, float *out);
+/* In place on sinograms g [nz][ntheta][n] (n even, at most 4096): the stripe removal of
+ * tao.py:58-71 with its beta and integer mask_size (columns [n/2 - ms/2, n/2 + ms/2),
+ * ms = mask_size * n, under Python's slice rules). */
+int rtv_prep_stripe_ti(float *g, int nz, int ntheta, int n, float beta, int mask_size);
+/* In place on projections proj [ntheta][ny][nz]: single-material phase retrieval
+ * (Paganin et al. 2002) as tomopy's retrieve_phase with pad=True applies it:
+ * edge-extended to power-of-two-type sizes, multiplied in 2D Fourier space by the
+ * normalised filter 1/(lambda dist |k|^2/(4 pi) + alpha), cropped.  pixel_size and
+ * dist in cm, energy in keV; padded sizes must factor into 2, 3, 5, 7 (<= 4096). */
+int rtv_prep_paganin(float *proj, int ntheta, int ny, int nz, float pixel_size, float dist, float energy,
+                     float alpha);
+
 #ifdef __cplusplus
 }
 #endif

@@ -629,3 +629,99 @@ if __name__ == "__main__":  # tiny self-run
     d = np.random.default_rng(0).random((nz, nt, n)).astype(np.float32)
     print(check_adjoints(d, th, ph, n, nt, m, nz, nz, 4.0, n / 2))
     print(run(d, th, ph, n, nt, m, nz, 1, n / 2, 1e-3 * 4, 4.0, 2, 2, log=sys.stderr).shape)
+
+
+# ---------------------------------------------------------------------------
+# Pre-processing before the solver (examples/tao/tao.py, examples/tao/rec_line.txt)
+# ---------------------------------------------------------------------------
+
+def darkflat_log(data, dark, flat):
+    """-log((data - mean(dark)) / (1e-3 + mean(flat) - mean(dark))) with the frame
+    means over axis 0, then swapaxes(0, 1) to sinograms (examples/tao/tao.py:34-43).
+    data [ntheta, nz, n], dark [ndark, nz, n], flat [nflat, nz, n] -> [nz, ntheta, n]."""
+    d = np.asarray(data, np.float64)
+    dm = np.mean(np.asarray(dark, np.float64), axis=0)
+    fm = np.mean(np.asarray(flat, np.float64), axis=0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        out = -np.log((d - dm) / (1e-3 + fm - dm))
+    return np.ascontiguousarray(out.swapaxes(0, 1))
+
+
+def _py_slice_bounds(lo, hi, n):
+    """Bounds of a[lo:hi] for a length-n axis under Python's slice rules."""
+    return slice(lo, hi).indices(n)[:2]
+
+
+def remove_stripe_ti(g, beta, mask_size):
+    """Stripe removal of examples/tao/tao.py:58-71 (after V. Titarenko) on sinograms
+    g [nz, ntheta, n] (the example applies it to data.swapaxes(0, 1), i.e. with the
+    mean taken over the projections): gamma_k = beta ((1-beta)/(1+beta))^|k| over the
+    signed detector frequencies k (gamma_0 - 1), v = mean over projections minus its
+    first column, v <- v (*) gamma (circular convolution via rfft/irfft), and v is
+    added to every projection on the columns [n//2 - ms//2, n//2 + ms//2) with
+    ms = mask_size * n (integer mask_size, Python slice rules).  Even n."""
+    g = np.array(g, np.float64)
+    n = g.shape[-1]
+    gamma = beta * ((1 - beta) / (1 + beta)) ** np.abs(np.fft.fftfreq(n) * n)
+    gamma[0] -= 1
+    v = np.mean(g, axis=1)
+    v = v - v[:, 0:1]
+    v = np.fft.irfft(np.fft.rfft(v) * np.fft.rfft(gamma), n)
+    ms = int(mask_size) * n
+    lo, hi = _py_slice_bounds(n // 2 - ms // 2, n // 2 + ms // 2, n)
+    mask = np.zeros(n)
+    mask[lo:hi] = 1
+    return g + (v * mask)[:, None, :]
+
+
+# Physical constants of tomopy's phase retrieval (tomopy/prep/phase.py, tomopy 1.x)
+_PLANCK_KEV_S = 6.58211928e-19  # reduced Planck constant [keV s]
+_SPEED_OF_LIGHT_CM = 299792458e+2  # [cm/s]
+_PI_TOMOPY = 3.14159265359
+
+
+def wavelength_cm(energy_kev):
+    """2 pi hbar c / E [cm] (tomopy's _wavelength)."""
+    return 2 * _PI_TOMOPY * _PLANCK_KEV_S * _SPEED_OF_LIGHT_CM / energy_kev
+
+
+def paganin_pad_width(dim, pixel_size, dist, energy):
+    """Pad width of tomopy's _calc_pad_width: half the distance to the next power of two
+    above dim + ceil(pi lambda dist / pixel^2)."""
+    pad_pix = np.ceil(_PI_TOMOPY * wavelength_cm(energy) * dist / pixel_size ** 2)
+    return int((pow(2, np.ceil(np.log2(dim + pad_pix))) - dim) * 0.5)
+
+
+def _reciprocal_coord(pixel_size, num_grid):
+    n = num_grid - 1
+    rc = np.arange(-n, num_grid, 2, dtype=np.float64)
+    return rc * (2 * _PI_TOMOPY / (n * pixel_size))
+
+
+def paganin_filter(ny, nz, pixel_size, dist, energy, alpha):
+    """Normalised single-material phase filter (Paganin et al., J. Microsc. 206 (2002)
+    33-40) as tomopy's retrieve_phase forms it on a padded ny x nz projection:
+    H = 1/(lambda dist w2/(4 pi) + alpha) on the grid w2 = kx^2 + ky^2 of
+    _reciprocal_grid, fftshift-ed and divided by its maximum."""
+    w2 = np.add.outer(_reciprocal_coord(pixel_size, ny) ** 2, _reciprocal_coord(pixel_size, nz) ** 2)
+    h = np.fft.fftshift(1.0 / (wavelength_cm(energy) * dist * w2 / (4 * _PI_TOMOPY) + alpha))
+    return h / h.max()
+
+
+def retrieve_phase(proj, pixel_size=1e-4, dist=50.0, energy=20.0, alpha=1e-3):
+    """Paganin phase retrieval of every projection proj[k] (proj [ntheta, ny, nz],
+    intensities after dark/flat normalisation), restating tomopy's retrieve_phase with
+    pad=True: the projection is edge-extended by the pad widths of paganin_pad_width on
+    each side, multiplied in 2D Fourier space by paganin_filter and cropped back.
+    Units: pixel_size and dist in cm, energy in keV (tomocupy's command line of
+    examples/tao/rec_line.txt:1 gives 3.13 um, 250 mm, 25 keV, alpha 5e-4)."""
+    proj = np.asarray(proj, np.float64)
+    nt, ny, nz = proj.shape
+    py = paganin_pad_width(ny, pixel_size, dist, energy)
+    pz = paganin_pad_width(nz, pixel_size, dist, energy)
+    H = paganin_filter(ny + 2 * py, nz + 2 * pz, pixel_size, dist, energy, alpha)
+    out = np.empty_like(proj)
+    for k in range(nt):
+        p = np.pad(proj[k], ((py, py), (pz, pz)), mode="edge")
+        out[k] = np.real(sfft.ifft2(sfft.fft2(p) * H))[py:py + ny, pz:pz + nz]
+    return out

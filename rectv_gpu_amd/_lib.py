@@ -52,6 +52,9 @@ SIGNATURES = {
     "rtv_autotune": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "rtv_batch_iterate": (_I, [ctypes.POINTER(_P), _I, _P, _P, _P, _P]),
     "rtv_footprint": (_I, [_I, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_longlong)]),
+    "rtv_prep_darkflat": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P]),
+    "rtv_prep_stripe_ti": (_I, [_P, _I, _I, _I, _F, _I]),
+    "rtv_prep_paganin": (_I, [_P, _I, _I, _I, _F, _F, _F, _F]),
 }
 
 

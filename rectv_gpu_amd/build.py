@@ -27,7 +27,7 @@ def hipcc():
 
 
 def build(verbose=True):
-    srcs = [os.path.join(CSRC, "solver.hip")]
+    srcs = [os.path.join(CSRC, "solver.hip"), os.path.join(CSRC, "prep.hip")]
     tmp = LIB + ".tmp"
     cmd = [hipcc()] + FLAGS + ["-I", os.path.join(HERE, "..", "include")] + srcs + ["-o", tmp]
     if verbose:

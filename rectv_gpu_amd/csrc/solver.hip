@@ -19,7 +19,7 @@
 
 namespace rtv {
 
-static thread_local std::string g_last_error;
+thread_local std::string g_last_error;  // message of the last failing C-ABI call (all sources)
 
 static const double PI_REF = 3.1415926535;  // kernels_radonusfft.cuh:1
 static const double PI_EXACT = 3.14159265358979323846;
