@@ -162,6 +162,34 @@ def cpu_baseline(w, titer, threads=None):
                                                                 os.cpu_count(), titer)}
 
 
+MIX_FIXTURE = os.path.join(ROOT, "tests", "golden", "mix_samples.npz")
+
+
+def oracle_sample_errors(jobs, mix):
+    """Relative L2 error of the sampled jobs' results against the FP64 oracle results
+    committed in tests/golden/mix_samples.npz (a strided subsample of each result,
+    made by tests/golden/make_mix_samples.py for the 64-job mixes at niter=64,
+    titer=4), or None when the fixture does not cover this run.  Jobs whose stop
+    iteration differs from the oracle's are reported, not compared."""
+    if not os.path.exists(MIX_FIXTURE) or len(jobs) != 64:
+        return None
+    z = np.load(MIX_FIXTURE, allow_pickle=False)
+    ids = sorted({int(k.split("_")[1]) for k in z.files if k.startswith(mix + "_")})
+    out = {}
+    for i in ids:
+        j, pre = jobs[i], "%s_%d_" % (mix, i)
+        if j.niter != 64 or j.titer != 4 or j.result is None:
+            return None
+        if j.iterations != int(z[pre + "iterations"]):
+            out[j.name] = "stopped after %d, oracle after %d" % (j.iterations, int(z[pre + "iterations"]))
+            continue
+        zs, xs = max(1, -(-j.nz // 4)), max(1, j.n // 16)
+        sub = j.result[::zs, :, ::xs, ::xs].astype(np.float64)
+        ref = z[pre + "f_sub"].astype(np.float64)
+        out[j.name] = float(np.linalg.norm(sub - ref) / np.linalg.norm(ref))
+    return out
+
+
 def batched_bench(count, niter, titer, workers=8):
     """Batched-instances mix: `count` uneven jobs solved one after another with the
     plain API (Solver.recon each) and then together (batch.solve_batch); same
@@ -191,7 +219,8 @@ def batched_bench(count, niter, titer, workers=8):
                         "projections, M {4,8,16}, Nzp=Nz, niter=%d, titer=%d" % (count, niter, titer),
             "value": round(vox / t_bat, 1), "unit": "voxel-iter/s", "sequential_value": round(vox / t_seq, 1),
             "speedup_vs_sequential": round(t_seq / t_bat, 3), "batched_s": round(t_bat, 3),
-            "sequential_s": round(t_seq, 3), "voxel_iterations": vox, "max_rel_l2_vs_plain": diff}
+            "sequential_s": round(t_seq, 3), "voxel_iterations": vox, "max_rel_l2_vs_plain": diff,
+            "sample_rel_l2_vs_fp64_oracle": oracle_sample_errors(jobs_b, "batch")}
 
 
 def served_bench(count, niter, titer, rate, seed=7, max_active=16, sample_frac=0.1):
@@ -199,7 +228,10 @@ def served_bench(count, niter, titer, rate, seed=7, max_active=16, sample_frac=0
     arriving as a Poisson process of `rate` jobs/s, admitted by device footprint,
     iterations interleaved.  Reports throughput over the whole run, p50/p99 job
     latency (arrival to result on the host), and, on a sample of the jobs, the
-    relative L2 difference of the served result from a plain solve of that job."""
+    relative L2 difference of the served result from a plain solve of the same
+    number of iterations and from the FP64 oracle (committed fixture).  Each job is
+    checked after every outer iteration (ahead=1), so it stops at the first
+    iteration below the tolerance, as the oracle's run does."""
     import threading
     from rectv_gpu_amd import synthetic, batch
     specs = synthetic.serve_specs(count)
@@ -207,7 +239,7 @@ def served_bench(count, niter, titer, rate, seed=7, max_active=16, sample_frac=0
     jobs = [synthetic.make_job(sp, niter, titer, tol=1e-4) for sp in specs]
     gaps = np.random.default_rng(seed).exponential(1.0 / rate, count)
     print("bench: serving mix of %d jobs generated" % count, file=sys.stderr, flush=True)
-    srv = batch.Server(max_active=max_active, ahead=2, workers=4)
+    srv = batch.Server(max_active=max_active, ahead=1, workers=4)
 
     def producer():
         t0 = time.perf_counter()
@@ -236,13 +268,18 @@ def served_bench(count, niter, titer, rate, seed=7, max_active=16, sample_frac=0
     sample = sorted(range(count), key=lambda i: jobs[i].voxels)[:k]
     diff = 0.0
     for i in sample:
-        r = batch.solve_sequential([jobs[i].fresh()])[0].result
+        plain = jobs[i].fresh()
+        plain.niter = jobs[i].iterations  # the served job may have stopped early
+        r = batch.solve_sequential([plain])[0].result
         diff = max(diff, float(np.linalg.norm(jobs[i].result.astype(np.float64) - r) / max(np.linalg.norm(r), 1e-30)))
     return {"workload": "%d jobs (batched mix + 5%% N=1024 Nz=32 M=16), Poisson arrivals %.0f jobs/s, niter=%d "
                         "or relative change < 1e-4, titer=%d, max %d active" % (count, rate, niter, titer, max_active),
             "value": round(vox / (t1 - t0), 1), "unit": "voxel-iter/s", "wall_s": round(t1 - t0, 3),
             "latency": lat, "jobs": count, "errors": len(errors), "peak_active": srv.peak_active,
-            "sample_max_rel_l2_vs_plain": diff, "sample_jobs": k}
+            "sample_max_rel_l2_vs_plain": diff, "sample_jobs": k,
+            "iterations": {"min": min(j.iterations for j in jobs), "max": max(j.iterations for j in jobs),
+                           "stopped_early": sum(1 for j in jobs if j.iterations < j.niter)},
+            "sample_rel_l2_vs_fp64_oracle": oracle_sample_errors(jobs, "serve")}
 
 
 def main():
@@ -260,7 +297,7 @@ def main():
     ap.add_argument("--mode", default="all", choices=["all", "plain", "batched", "served"],
                     help="plain: the headline solve only; batched / served: those forms only; all: each")
     ap.add_argument("--jobs", type=int, default=64, help="jobs in the batched / served mixes")
-    ap.add_argument("--job-niter", type=int, default=16, help="outer iterations per batched / served job")
+    ap.add_argument("--job-niter", type=int, default=64, help="outer iterations per batched / served job")
     ap.add_argument("--job-titer", type=int, default=4)
     ap.add_argument("--rate", type=float, default=20.0, help="serving arrival rate (jobs/s)")
     ap.add_argument("--light-rate", type=float, default=2.0,

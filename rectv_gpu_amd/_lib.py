@@ -8,7 +8,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "librectv_amd.so")
+# RECTV_LIB selects another build of the same library (e.g. one compiled with other
+# build-time tuning constants); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("RECTV_LIB") or os.path.join(_HERE, "librectv_amd.so")
 
 _lib = None
 

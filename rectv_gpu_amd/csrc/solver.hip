@@ -252,6 +252,23 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             cnt[keys[a].k >> 32]++;
         }
         for (int t = 0; t < ntile; t++) off[t + 1] = off[t] + cnt[t];
+        // Spread each tile's records over its chunks round robin (record k of the
+        // sorted list goes to chunk k mod nchunks): a chunk then covers the whole tile
+        // evenly, so the 16 waves (one 4 x 4 cell block each) get lists of similar
+        // length between two barriers.  Sorted chunks put nearby window origins
+        // together and leave most waves idle (busiest wave ~2.6x the mean at the full
+        // workload, 1.3x interleaved).
+        {
+            std::vector<int> tmp;
+            for (int t = 0; t < ntile; t++) {
+                const int c = cnt[t], nchk = (c + SM_CH - 1) / SM_CH;
+                if (nchk < 2) continue;
+                tmp.assign(hg.asids.begin() + off[t], hg.asids.begin() + off[t] + c);
+                int q = off[t];
+                for (int ch = 0; ch < nchk; ch++)
+                    for (int k = ch; k < c; k += nchk) hg.asids[q++] = tmp[k];
+            }
+        }
         hg.aitems.resize(hg.asids.size());
         hg.arec.resize(hg.asids.size());
         const int MPh = (M + 1) / 2;
