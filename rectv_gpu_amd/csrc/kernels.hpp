@@ -307,7 +307,8 @@ constexpr int GM_ITEMS = 256;  // items per workgroup: 16 waves x 16
 constexpr int GM_THREADS = 1024;
 constexpr int GM_KS = 4;       // max steps of 4 cells per block (the host packs blocks so)
 constexpr int GM_ZG = 8;       // default slices per workgroup (runtime argument zg)
-constexpr int GM_WYP = 16;     // padded row-tap entries per item (3 zeros, 9 taps, 4 zeros)
+constexpr int GM_WYP = 17;     // row-tap entries per item (3 zeros, 9 taps, 5 zeros); odd, so
+                               // that the 16 items of a block read 16 different LDS banks
 template <int NB>              // 16-column blocks: MP <= 8*NB
 __global__ void __launch_bounds__(GM_THREADS)
 k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
@@ -713,7 +714,12 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
     constexpr int NC = 16 * NB;
     constexpr int HALF = 8 * NB;       // column of the imaginary part of pair 0
-    constexpr int NCP = NC + 1;        // padded row of P
+    // P rows: NC columns, unpadded.  The host orders each block's list in groups of
+    // 4 slots that are distinct mod 4 wherever it can, so one matrix-core operand
+    // read (4 slots x 16 columns) meets 4 different 16-bank groups; for NB = 2 the
+    // column halves of slots with bit 1 set are swapped (col ^ 16), which the reads
+    // undo per lane.  The weight rows (16 floats per slot) fall likewise.
+    constexpr int NCP = NC;
     constexpr int NPP = HALF / 8;      // pairs per staging thread
     constexpr int NS = SM_CH + 1;      // slots incl. the zero slot
     __shared__ float wyr[NS][TS];
@@ -747,73 +753,81 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
     // chunk staging data are prefetched into registers one chunk ahead, so the
     // dependent global loads (item -> q, coefficients) overlap the MFMA loop; the
     // weight rows come precomputed (k_scatter_weights), one float4 per thread
+    // Every load below is unconditional (indices clamped into range; values past a
+    // chunk's end are staged but never listed), so that the compiler's wait counts
+    // stay exact and the loads of the next chunk remain in flight through the
+    // matrix-core loop.  The record of a slot is loaded two chunks ahead, so the
+    // dependent loads (q, coefficients) can be issued at once one chunk ahead.
     bool has = false;  // this thread's slot holds a record of the chunk
+    bool mir = false;  // ... and that record is a mirrored sample
     float4 pw = make_float4(0.f, 0.f, 0.f, 0.f);
     const size_t sq = (size_t)nt * n;  // float2 per slice of q
     float2 pq[ZW], pph[NPP];
-    constexpr int NLB = (SM_LB + 4 * 1024 - 1) / (4 * 1024);  // list words per thread
-    unsigned plw[NLB];
-    int2 pcr = make_int2(0, 0);
-    auto prefetch = [&](int c0) {
+    constexpr int NLW = SM_LB / 4;     // list words per chunk staging (blist is padded by SM_LB bytes)
+    static_assert(NLW <= 1024, "one list word per thread");
+    unsigned plw = 0u;
+    int2 nrec = make_int2(0, 0);       // record of slot sm in the chunk after the prefetched one
+    int2 ncr = make_int2(0, 0);        // list range of that chunk
+    auto rec_of = [&](int c0) {
         const int cnt = min(SM_CH, te.count - c0);
-        has = sm < cnt;
-        if (has) {
-            // slot sm's weight rows (8 float4 per slot: thread tid holds part tid & 7)
-            pw = wtab[(size_t)(te.pad + c0 / SM_CH) * SM_CH * 8 + tid];
-            // record: item 2 * sample | mirror flag, coefficient row
-            const int2 rec = arec[te.start + c0 + sm];
-            const unsigned qb = 4u * (unsigned)(rec.x & ~1);  // byte offset of the sample's q in a slice
-            const bool mir = rec.x & 1;
-#pragma unroll
-            for (int zz = 0; zz < ZW; zz++) {
-                float2 q = zz < zc ? *reinterpret_cast<const float2 *>(
-                                         reinterpret_cast<const char *>(qs + (size_t)(z0 + zz) * sq) + qb)
-                                   : make_float2(0.f, 0.f);
-                if (mir) q.y = -q.y;
-                pq[zz] = q;
-            }
-#pragma unroll
-            for (int q = 0; q < NPP; q++) {
-                const int j = part + 8 * q;
-                pph[q] = j < MP ? cf[rec.y + j] : make_float2(0.f, 0.f);
-            }
-        }
-        // the chunk's block lists: words [pcr.x/4, (pcr.x + pcr.y)/4) of blist (4-byte aligned)
-        pcr = crange[te.pad + c0 / SM_CH];
-        const unsigned *lw = reinterpret_cast<const unsigned *>(blist + pcr.x);
-#pragma unroll
-        for (int q = 0; q < NLB; q++) {
-            const int wi = tid + q * 1024;
-            plw[q] = 4 * wi < pcr.y ? lw[wi] : 0u;
-        }
+        return arec[te.start + c0 + min(sm, cnt - 1)];
     };
-    if (te.count > 0) prefetch(0);
+    auto prefetch = [&](int c0, int2 rec, int2 cr) {
+        const int ci = te.pad + c0 / SM_CH;
+        has = sm < min(SM_CH, te.count - c0);
+        mir = rec.x & 1;
+        // slot sm's weight rows (8 float4 per slot: thread tid holds part tid & 7)
+        pw = wtab[(size_t)ci * SM_CH * 8 + tid];
+        // record: item 2 * sample | mirror flag, coefficient row
+        const unsigned qb = 4u * (unsigned)(rec.x & ~1);  // byte offset of the sample's q in a slice
+#pragma unroll
+        for (int zz = 0; zz < ZW; zz++)
+            pq[zz] = *reinterpret_cast<const float2 *>(
+                reinterpret_cast<const char *>(qs + (size_t)min(z0 + zz, nzb - 1) * sq) + qb);
+#pragma unroll
+        for (int q = 0; q < NPP; q++) pph[q] = cf[rec.y + min(part + 8 * q, MP - 1)];
+        // the chunk's block lists: SM_LB bytes from its first (4-byte aligned) list
+        plw = reinterpret_cast<const unsigned *>(blist + __builtin_amdgcn_readfirstlane(cr.x))[min(tid, NLW - 1)];
+    };
+    if (te.count > 0) {
+        const int2 r0 = rec_of(0);
+        prefetch(0, r0, crange[te.pad]);
+        if (SM_CH < te.count) {
+            nrec = rec_of(SM_CH);
+            ncr = crange[te.pad + 1];
+        }
+    }
     for (int c0 = 0; c0 < te.count; c0 += SM_CH) {
         // this wave's list within the chunk: (byte offset, length, a multiple of 4)
-        const int2 lst = brange[((size_t)te.pad + c0 / SM_CH) * 16 + w];
+        const int2 lst = brange[((size_t)te.pad + c0 / SM_CH) * 16 + __builtin_amdgcn_readfirstlane(w)];
         lds_barrier();
-#pragma unroll
-        for (int q = 0; q < NLB; q++) {
-            const int wi = tid + q * 1024;
-            if (4 * wi < pcr.y) reinterpret_cast<unsigned *>(sl)[wi] = plw[q];
-        }
+        if (tid < NLW) reinterpret_cast<unsigned *>(sl)[tid] = plw;
         if (has) {
             // weight rows: parts 0-3 hold wy[4 part ..], parts 4-7 wx[4 (part - 4) ..]
             float *wd = part < 4 ? &wyr[sm][4 * part] : &wxr[sm][4 * (part - 4)];
             *reinterpret_cast<float4 *>(wd) = pw;
 #pragma unroll
             for (int zz = 0; zz < ZW; zz++) {
+                const float2 qv = make_float2(pq[zz].x, mir ? -pq[zz].y : pq[zz].y);
 #pragma unroll
                 for (int q = 0; q < NPP; q++) {
                     const int j = part + 8 * q;
-                    const float2 p = cmulc(pq[zz], pph[q]);  // q' * conj(cf); zero for j >= MP
-                    sP[zz][sm][j] = p.x;
-                    sP[zz][sm][HALF + j] = p.y;
+                    // q' * conj(cf); zero for pairs j >= MP
+                    const float2 p = j < MP ? cmulc(qv, pph[q]) : make_float2(0.f, 0.f);
+                    const int sw = NB == 2 ? 16 * ((sm >> 1) & 1) : 0;
+                    sP[zz][sm][j ^ sw] = p.x;
+                    sP[zz][sm][(HALF + j) ^ sw] = p.y;
                 }
             }
         }
         // slots of the chunk past its item count are never listed
-        if (c0 + SM_CH < te.count) prefetch(c0 + SM_CH);
+        if (c0 + SM_CH < te.count) {
+            prefetch(c0 + SM_CH, nrec, ncr);
+            if (c0 + 2 * SM_CH < te.count) {
+                nrec = rec_of(c0 + 2 * SM_CH);
+                ncr = crange[te.pad + c0 / SM_CH + 2];
+            }
+        }
         lds_barrier();
         const unsigned char *ls = sl + lst.x + lk;
         const int len = __builtin_amdgcn_readfirstlane(lst.y);
@@ -821,12 +835,13 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
         for (int k0 = 0; k0 < len; k0 += 4) {
             const int slot = ls[k0];
             const float av = wyr[slot][ar] * wxr[slot][ac];
-            const float *pp = &sP[0][slot][li];
+            const int sh = NB == 2 ? (slot >> 1) & 1 : 0;  // swapped halves
+            const float *pp = &sP[0][slot][li + 16 * sh];
 #pragma unroll
             for (int zz = 0; zz < ZW; zz++) {
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
-                    const float bv = pp[zz * NS * NCP + b * 16];
+                    const float bv = pp[zz * NS * NCP + b * (16 - 32 * sh)];
                     acc[zz][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[zz][b], 0, 0, 0);
                 }
             }

@@ -304,10 +304,30 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
                 for (int b = 0; b < 16; b++) {
                     const int r0 = 4 * (b / 4), q0 = 4 * (b % 4);
                     const int off = (int)hg.blist.size() - base;
+                    // the block's slots by residue mod 4, emitted in groups of 4 whose
+                    // k-th entry is k mod 4 while the residues last (distinct LDS bank
+                    // groups for the 4 slots one matrix-core operand read covers)
+                    std::vector<int> byres[4];
                     for (int i = 0; i < cn; i++)
                         if (ly[i] <= r0 + 3 && ly[i] + KW - 1 >= r0 && lx[i] <= q0 + 3 && lx[i] + KW - 1 >= q0)
-                            hg.blist.push_back((unsigned char)i);
-                    while (((int)hg.blist.size() - base - off) % 4) hg.blist.push_back((unsigned char)SM_CH);
+                            byres[i & 3].push_back(i);
+                    size_t head[4] = {0, 0, 0, 0};
+                    auto left = [&](int k) { return byres[k].size() - head[k]; };
+                    for (;;) {
+                        int k0 = 0;
+                        for (int k = 1; k < 4; k++)
+                            if (left(k) > left(k0)) k0 = k;
+                        if (left(k0) == 0) break;
+                        for (int k = 0; k < 4; k++) {
+                            int src = k;
+                            if (left(src) == 0) {  // residue k exhausted: the fullest other one
+                                src = 0;
+                                for (int j = 1; j < 4; j++)
+                                    if (left(j) > left(src)) src = j;
+                            }
+                            hg.blist.push_back((unsigned char)(left(src) ? byres[src][head[src]++] : SM_CH));
+                        }
+                    }
                     hg.brange.push_back(make_int2(off, (int)hg.blist.size() - base - off));
                 }
                 const int len = (int)hg.blist.size() - base;
@@ -316,6 +336,8 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
                 hg.achunk.push_back(make_int4(t, te.start + c0, cn, 0));
             }
         }
+        // the scatter stages SM_LB bytes from each chunk's first list
+        hg.blist.insert(hg.blist.end(), SM_LB, (unsigned char)SM_CH);
     }
     // G cells inside some window (direct or mirrored), per row of G: the shortest
     // circular run of columns holding them.  The column passes skip the other
