@@ -101,6 +101,9 @@ static int usfft_K(int n, float mu) {
     return (int)std::ceil(2.0 * n * 1 / PI_REF * std::sqrt(-m * std::log(eps) + (m * n) * (m * n) / 4));
 }
 
+#ifndef RTV_GM_BALANCE
+#define RTV_GM_BALANCE 1  // gather block placement (build-time constant): 0 in order, 1 / 2 balanced
+#endif
 static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *theta, const float *phi, float center) {
     const int N = 2 * n, nT = N / TS, K = 4;
     const float mu = usfft_mu(n);
@@ -206,7 +209,9 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             }
             close_block();
             const int e1 = (int)hg.fitems.size();
-            // the kernel relies on every block spanning at most 12 rows and 4 steps of 4 cells
+            // the kernel relies on every block spanning at most 12 rows and 4 steps of 4 cells;
+            // a block's matrix-core work is rows x steps instructions per column block
+            std::vector<int> cost;
             for (int b0 = e0; b0 < e1; b0 += 16) {
                 int r0 = 1 << 30, r1 = -1, c0 = 1 << 30, c1 = -1;
                 for (int i = b0; i < b0 + 16; i++) {
@@ -218,12 +223,52 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
                     c0 = std::min(c0, lx0);
                     c1 = std::max(c1, lx0);
                 }
+                cost.push_back(r1 < 0 ? 0 : (r1 - r0 + KW) * ((c1 + KW - 1 - (c0 & ~3)) / 4 + 1));
                 if (r1 < 0) continue;
                 if (r1 - r0 + KW > 12 || (c1 + KW - 1 - (c0 & ~3)) / 4 + 1 > GM_KS || r1 >= TS || c1 >= TS)
                     throw Error(-1, "internal: gather block exceeds its window bounds");
             }
+#if RTV_GM_BALANCE
+            // Each entry's blocks (one per wave) are placed so that the four SIMDs of the
+            // CU get similar work between the kernel's per-slice barriers: longest
+            // block first onto the least loaded SIMD (4 waves each), wave w running on
+            // SIMD w mod 4.  Unused wave slots before the last used one hold empty
+            // blocks.  Results do not depend on the placement.
+            {
+                std::vector<int> blk(hg.fitems.begin() + e0, hg.fitems.end());
+                hg.fitems.resize(e0);
+                const int nb = (int)cost.size();
+                for (int q0 = 0; q0 < nb; q0 += GM_ITEMS / 16) {
+                    const int q1 = std::min(nb, q0 + GM_ITEMS / 16);
+                    std::vector<int> order;
+                    for (int q = q0; q < q1; q++) order.push_back(q);
+                    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+                    int load[4] = {0, 0, 0, 0}, used[4] = {0, 0, 0, 0};
+                    int slot[GM_ITEMS / 16];
+                    std::fill(slot, slot + GM_ITEMS / 16, -1);
+                    for (int q : order) {
+                        int s = -1;
+                        for (int k = 0; k < 4; k++)
+                            if (used[k] < GM_ITEMS / 64 && (s < 0 || load[k] < load[s])) s = k;
+                        load[s] += cost[q];
+#if RTV_GM_BALANCE == 2
+                        slot[s * (GM_ITEMS / 64) + used[s]++] = q;
+#else
+                        slot[s + 4 * used[s]++] = q;
+#endif
+                    }
+                    int last = GM_ITEMS / 16 - 1;
+                    while (last > 0 && slot[last] < 0) last--;
+                    const int s0 = (int)hg.fitems.size();
+                    for (int w = 0; w <= last; w++)
+                        for (int i = 0; i < 16; i++) hg.fitems.push_back(slot[w] < 0 ? -1 : blk[16 * slot[w] + i]);
+                    hg.fmtiles.push_back(TileEntry{t, s0, (int)hg.fitems.size() - s0, 0});
+                }
+            }
+#else
             for (int s0 = e0; s0 < e1; s0 += GM_ITEMS)
                 hg.fmtiles.push_back(TileEntry{t, s0, std::min(GM_ITEMS, e1 - s0), 0});
+#endif
         }
     }
     // adjoint lists: every tile the 9x9 window intersects, each tile's items
