@@ -309,15 +309,19 @@ constexpr int GM_KS = 4;       // max steps of 4 cells per block (the host packs
 constexpr int GM_ZG = 8;       // default slices per workgroup (runtime argument zg)
 constexpr int GM_WYP = 17;     // row-tap entries per item (3 zeros, 9 taps, 5 zeros); odd, so
                                // that the 16 items of a block read 16 different LDS banks
-template <int NB>              // 16-column blocks: MP <= 8*NB
+// More than 16 pairs (M > 32) run as one launch per group of at most 8*NB pairs
+// p0 .. p0 + 8*NB - 1 of the MPt pairs: the item sums are linear in the pairs, so
+// launches after the first add their part to gq (accum != 0).
+template <int NB>              // 16-column blocks: pairs of this launch <= 8*NB
 __global__ void __launch_bounds__(GM_THREADS)
 k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
               const int *__restrict__ items, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf,
-              int n, int nt, int MP, float beta, int nentries, int nzb, int zg) {
+              int n, int nt, int MPt, int p0, int accum, float beta, int nentries, int nzb, int zg) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
     constexpr int NC = 16 * NB, HALF = 8 * NB, NCP = NC + 1;
     constexpr int CELLS = TH * TH;
     constexpr int NLD = (HALF * CELLS + GM_THREADS - 1) / GM_THREADS;  // tile loads per thread and slice
+    const int MP = min(HALF, MPt - p0);  // pairs of this launch
     // 4 spare cells: the last 4-cell step of row TH-1 may read up to 3 cells past
     // the region (their weights are zero; the spare cells are zeroed)
     __shared__ float tile[(CELLS + 4) * NCP];
@@ -357,7 +361,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     }
     float2 stv[NLD];
     auto load_tile = [&](int z) {
-        const char *Gz = reinterpret_cast<const char *>(G + (size_t)z * MP * N * N);
+        const char *Gz = reinterpret_cast<const char *>(G + ((size_t)z * MPt + p0) * N * N);
 #pragma unroll
         for (int q = 0; q < NLD; q++) stv[q] = *reinterpret_cast<const float2 *>(Gz + tofs[q]);
     };
@@ -433,7 +437,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     const int oid = b0 + li < cnt ? sid[b0 + li] : -1;
     float2 cfv[NB][4];
     {
-        const float2 *crow = cf + (oid >= 0 ? ((size_t)((oid >> 1) / n) * 2 + (oid & 1)) * MP : 0);
+        const float2 *crow = cf + (oid >= 0 ? ((size_t)((oid >> 1) / n) * 2 + (oid & 1)) * MPt + p0 : 0);
 #pragma unroll
         for (int b = 0; b < NB; b++)
 #pragma unroll
@@ -522,7 +526,15 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
                 }
             sx = add_xor32(add_xor16(sx), bp32);
             sy = add_xor32(add_xor16(sy), bp32);
-            if (lk == 0 && oid >= 0) out[(size_t)zz * nt * n * 2 + oid] = make_float2(sx, sy);
+            if (lk == 0 && oid >= 0) {
+                float2 *o = out + (size_t)zz * nt * n * 2 + oid;
+                if (accum) {
+                    const float2 pv = *o;
+                    sx += pv.x;
+                    sy += pv.y;
+                }
+                *o = make_float2(sx, sy);
+            }
         }
     }
 }
@@ -710,8 +722,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__restrict__ arec,
           const TileEntry *__restrict__ tiles, const int2 *__restrict__ crange,
           const int2 *__restrict__ brange, const unsigned char *__restrict__ blist, const float2 *__restrict__ cf,
-          const float4 *__restrict__ wtab, int n, int nt, int MP, int ntiles, int nzb) {
+          const float4 *__restrict__ wtab, int n, int nt, int MPt, int p0, int ntiles, int nzb) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
+    // this launch writes the G planes of pairs p0 .. p0 + MP - 1 of the MPt pairs
+    // (more than 16 pairs run as one launch per group)
+    const int MP = min(8 * NB, MPt - p0);
     constexpr int NC = 16 * NB;
     constexpr int HALF = 8 * NB;       // column of the imaginary part of pair 0
     // P rows: NC columns, unpadded.  The host orders each block's list in groups of
@@ -785,7 +800,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
             pq[zz] = *reinterpret_cast<const float2 *>(
                 reinterpret_cast<const char *>(qs + (size_t)min(z0 + zz, nzb - 1) * sq) + qb);
 #pragma unroll
-        for (int q = 0; q < NPP; q++) pph[q] = cf[rec.y + min(part + 8 * q, MP - 1)];
+        for (int q = 0; q < NPP; q++) pph[q] = cf[rec.y + p0 + min(part + 8 * q, MP - 1)];
         // the chunk's block lists: SM_LB bytes from its first (4-byte aligned) list
         plw = reinterpret_cast<const unsigned *>(blist + __builtin_amdgcn_readfirstlane(cr.x))[min(tid, NLW - 1)];
     };
@@ -864,7 +879,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
             }
         }
         if (zz < zc && li < MP) {
-            float4 *o = reinterpret_cast<float4 *>(Gt + ((size_t)(z0 + zz) * MP + li) * N * N);
+            float4 *o = reinterpret_cast<float4 *>(Gt + ((size_t)(z0 + zz) * MPt + p0 + li) * N * N);
             o[0] = make_float4(re[0], im[0], re[1], im[1]);
             o[1] = make_float4(re[2], im[2], re[3], im[3]);
         }
@@ -1092,9 +1107,10 @@ __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i
 // (div, kernels_operators.cuh:46-73) is computed once per outer iteration, after
 // the rho rule.  Thread per (chunk, y, x) column; the t and z neighbours are the
 // previous frame's / slice's own values and are carried.
+// Frames i0 .. i0 + MM - 1 (one launch per group of MM frames when M > MM; i0 even).
 template <int MM>
 __global__ void __launch_bounds__(TV_THREADS)
-k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *__restrict__ dchi, TvArgs a) {
+k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *__restrict__ dchi, TvArgs a, int i0) {
     const TvTile tt = tv_tile(a.n, a.n, a.chunk0);
     const int x = tt.x, y = tt.y, c = tt.c;
     const int n = a.n, M = a.M, nzp = a.nzp;
@@ -1102,6 +1118,12 @@ k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *_
     const size_t plane = (size_t)n * n, S = (size_t)M * plane;
     const size_t col = (size_t)y * n + x;
     const float r = *a.rho;
+    // the group's frames seen from frame i0: dual entries shift by i0 planes of
+    // (n+1)^2 (strides keep the full frame count), outputs by i0 frame planes
+    psi += (size_t)i0 * (n + 1) * (n + 1);
+    mu += (size_t)i0 * (n + 1) * (n + 1);
+    dchi += (size_t)i0 * plane;
+    const int Mg = M - i0;
     auto chi = [&](size_t d) {
         const float4 p = psi[d], m = mu[d];
         return make_float4(m.x - r * p.x, m.y - r * p.y, m.z - r * p.z, m.w - r * p.w);
@@ -1109,13 +1131,13 @@ k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *_
     float czc[MM];  // chi_z of the previous slice plane (ghost plane z' = 0 first)
 #pragma unroll
     for (int i = 0; i < MM; i++)
-        if (i < M) czc[i] = chi(dual_idx(a, c, 0, i + 1, y + 1, x + 1)).w;
+        if (i < Mg) czc[i] = chi(dual_idx(a, c, 0, i + 1, y + 1, x + 1)).w;
     for (int zl = 0; zl < nzp; zl++) {
-        float ctc = chi(dual_idx(a, c, zl + 1, 0, y + 1, x + 1)).z;  // ghost frame i' = 0
+        float ctc = chi(dual_idx(a, c, zl + 1, 0, y + 1, x + 1)).z;  // frame i0 - 1 (the ghost frame for i0 = 0)
         float *o = dchi + ((size_t)c * nzp + zl) * S + col;
 #pragma unroll
         for (int i = 0; i < MM; i++) {
-            if (i < M) {
+            if (i < Mg) {
                 const size_t d = dual_idx(a, c, zl + 1, i + 1, y + 1, x + 1);
                 const float4 c0 = chi(d);
                 const float cxm = mu[d - 1].x - r * psi[d - 1].x;
@@ -1139,10 +1161,11 @@ k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *_
 // previous slice's / frame's own values, so they are carried.  dchi and v are
 // read, and out written, as non-temporal streams: the caches keep fm, whose
 // x / y neighbours other lanes and rows read again.
+// Frames i0 .. i0 + MM - 1 (one launch per group of MM frames when M > MM; i0 even).
 template <int MM>
 __global__ void __launch_bounds__(TV_THREADS)
 k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__restrict__ out,
-         const float *__restrict__ dchi, const float2 *__restrict__ v, TvArgs a, int zv0) {
+         const float *__restrict__ dchi, const float2 *__restrict__ v, TvArgs a, int zv0, int i0) {
     const TvTile tt = tv_tile(a.n, a.n, a.chunk0);
     const int x = tt.x, y = tt.y, c = tt.c;
     const int n = a.n, M = a.M, nzp = a.nzp;
@@ -1166,9 +1189,9 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
         const auto ra = buf_rsrc(flgl ? A + (zb - 1) * S : fm + zb * S);
 #pragma unroll
         for (int i = 0; i < MM; i++) {
-            if (i < M) {
-                fz[i] = buf_ld(rf, oc, i * pb);
-                hzc[i] = rgs * (fz[i] - buf_ld(ra, oc, i * pb));
+            if (i0 + i < M) {
+                fz[i] = buf_ld(rf, oc, (i0 + i) * pb);
+                hzc[i] = rgs * (fz[i] - buf_ld(ra, oc, (i0 + i) * pb));
             }
         }
     }
@@ -1178,27 +1201,29 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
             const auto rn = buf_rsrc(zl + 1 < nzp ? fm + (zg + 1) * S : (flgr ? A + (zg + 1) * S : fm + zg * S));
 #pragma unroll
             for (int i = 0; i < MM; i++)
-                if (i < M) fzp[i] = buf_ld(rn, oc, i * pb);
+                if (i0 + i < M) fzp[i] = buf_ld(rn, oc, (i0 + i) * pb);
         }
         const auto rf = buf_rsrc(fm + zg * S);
         const auto rd = buf_rsrc(dchi + zg * S);
         const auto ro = buf_rsrc(out + zg * S);
         const auto rv = buf_rsrc(v ? (const void *)(v + (size_t)(zg - zv0) * ((M + 1) / 2) * n * n) : (const void *)dchi);
-        // t-gradient value on frame plane 0: E(t'=1) - E(t'=0) = 0
-        float htc = 0.f;
+        // t-gradient value on frame i0 - 1: E(t'=1) - E(t'=0) = 0 for i0 = 0; the
+        // frame after the group (t-gradient of its last frame)
+        float htc = i0 > 0 ? rgt * (fz[0] - buf_ld(rf, oc, (i0 - 1) * pb)) : 0.f;
+        const float fnx = i0 + MM < M ? buf_ld(rf, oc, (i0 + MM) * pb) : 0.f;
         float2 vp = make_float2(0.f, 0.f);
 #pragma unroll
         for (int i = 0; i < MM; i++) {
-            if (i < M) {
-                // R^* part: frame i of the adjoint pairs (re: even i, im: odd i)
-                if ((i & 1) == 0 && v) vp = buf_ld2_nt(rv, 2 * oc, (i >> 1) * 2 * pb);
+            if (i0 + i < M) {
+                // R^* part: frame i0 + i of the adjoint pairs (re: even, im: odd; i0 is even)
+                if ((i & 1) == 0 && v) vp = buf_ld2_nt(rv, 2 * oc, ((i0 + i) >> 1) * 2 * pb);
                 const float rpi = (i & 1) ? vp.y : vp.x;
                 const float f0 = fz[i];
-                const unsigned so = i * pb;
+                const unsigned so = (i0 + i) * pb;
                 const float fxp = buf_ld(rf, oxp, so), fxm = buf_ld(rf, oxm, so);
                 const float fyp = buf_ld(rf, oyp, so), fym = buf_ld(rf, oym, so);
                 // register indices stay compile-time constants (i is unrolled)
-                const float ftp = (i + 1 < M) ? fz[(i + 1 < MM) ? i + 1 : i] : f0;
+                const float ftp = (i0 + i + 1 < M) ? ((i + 1 < MM) ? fz[(i + 1 < MM) ? i + 1 : i] : fnx) : f0;
                 const float hx0 = rgs * (fxp - f0), hxm = rgs * (f0 - fxm);
                 const float hy0 = rgs * (fyp - f0), hym = rgs * (f0 - fym);
                 const float ht0 = rgt * (ftp - f0);
@@ -1212,7 +1237,7 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
         }
 #pragma unroll
         for (int i = 0; i < MM; i++)
-            if (i < M) fz[i] = fzp[i];
+            if (i0 + i < M) fz[i] = fzp[i];
     }
 }
 

@@ -619,7 +619,6 @@ struct Solver {
         RTV_REQUIRE(n > 0 && nt > 0 && M > 0 && nz > 0 && nzp > 0, "all sizes must be positive");
         RTV_REQUIRE(nz % nzp == 0, "nz must be a multiple of nzp (the reference silently drops the remainder)");
         RTV_REQUIRE(n % 8 == 0, "n must be a multiple of 8 (16-cell spectral tiles of the 2n grid)");
-        RTV_REQUIRE(M <= 32, "at most 32 basis functions are supported");
         nch = nz / nzp;
         MP = (M + 1) / 2;
         ngpus = std::max(1, std::min(ngpus_, nch));
@@ -905,33 +904,46 @@ struct Solver {
         ray(zc, gdat, nullptr, h1out, mode, partial);
     }
 
+    // Pairs per gather launch: at most 16 (two 16-column blocks), and few enough that
+    // the kernel's 32-bit byte offsets within one slice's pair planes stay in range.
+    int gather_group() const {
+        const unsigned long long plane = 8ull * (2ull * n) * (2ull * n);
+        return (int)std::max(1ull, std::min(16ull, 0xFFFFFFFFull / plane));
+    }
     void gather(int zc, int zg) {
         const int blocks = nfm_entries * ((zc + zg - 1) / zg);
-        if (blocks > 0)
+        if (blocks <= 0) return;
+        const int pg = gather_group();
+        for (int p0 = 0; p0 < MP; p0 += pg) {
+            const int np = std::min(pg, MP - p0), acc = p0 > 0 ? 1 : 0;
             launch("gather", [&] {
-                if (MP <= 8)
+                if (np <= 8)
                     hipLaunchKernelGGL(k_gather_mfma<1>, dim3(blocks), dim3(GM_THREADS), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc, zg);
+                                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, p0, acc, beta, nfm_entries, zc, zg);
                 else
                     hipLaunchKernelGGL(k_gather_mfma<2>, dim3(blocks), dim3(GM_THREADS), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, beta, nfm_entries, zc, zg);
+                                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, p0, acc, beta, nfm_entries, zc, zg);
             });
+        }
     }
 
+    // one launch per group of at most 16 pairs (one or two 16-column blocks)
     template <int ZW>
     void scatter_t(int zc, const float2 *cf) {
         const int blocks = na_tiles * ((zc + ZW - 1) / ZW);
-        launch("scatter", [&] {
-            // MP <= 16 (M <= 32): one or two 16-column blocks
-            if (MP <= 8)
-                hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
-                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, cf, d_wtab.p, n, nt, MP,
-                                   na_tiles, zc);
-            else
-                hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
-                                   d_atiles.p, d_crange.p, d_brange.p, d_blist.p, cf, d_wtab.p, n, nt, MP,
-                                   na_tiles, zc);
-        });
+        for (int p0 = 0; p0 < MP; p0 += 16) {
+            const int np = std::min(16, MP - p0);
+            launch("scatter", [&] {
+                if (np <= 8)
+                    hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
+                                       d_atiles.p, d_crange.p, d_brange.p, d_blist.p, cf, d_wtab.p, n, nt, MP, p0,
+                                       na_tiles, zc);
+                else
+                    hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
+                                       d_atiles.p, d_crange.p, d_brange.p, d_blist.p, cf, d_wtab.p, n, nt, MP, p0,
+                                       na_tiles, zc);
+            });
+        }
     }
     void scatter(int zc, int zw, const float2 *cf) {
         if (zw == 2) scatter_t<2>(zc, cf);
@@ -1002,9 +1014,11 @@ struct Solver {
     template <int MM>
     void launch_update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
         const dim3 grid(tv_blocks(n, ncb));
-        launch("update", [&] {
-            hipLaunchKernelGGL(k_update<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, nxt, d_dchi.p, d_v.p, ta, zv0);
-        });
+        for (int i0 = 0; i0 < M; i0 += MM)  // groups of MM frames (one group unless M > 32)
+            launch("update", [&] {
+                hipLaunchKernelGGL(k_update<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, nxt, d_dchi.p, d_v.p, ta, zv0,
+                                   i0);
+            });
     }
     void update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
         if (M <= 4) launch_update<4>(cur, A, nxt, ta, ncb, zv0);
@@ -1027,9 +1041,10 @@ struct Solver {
     template <int MM>
     void launch_divchi(const TvArgs &ta) {
         const dim3 grid(tv_blocks(n, nch));
-        launch("divchi", [&] {
-            hipLaunchKernelGGL(k_divchi<MM>, grid, dim3(TV_THREADS), 0, st, d_psi.p, d_mu.p, d_dchi.p, ta);
-        });
+        for (int i0 = 0; i0 < M; i0 += MM)
+            launch("divchi", [&] {
+                hipLaunchKernelGGL(k_divchi<MM>, grid, dim3(TV_THREADS), 0, st, d_psi.p, d_mu.p, d_dchi.p, ta, i0);
+            });
     }
     void divchi(const TvArgs &ta) {
         if (M <= 4) launch_divchi<4>(ta);
@@ -1630,7 +1645,6 @@ static void prox_noalloc(const float *fcur, const float *fprev, float *psi, floa
     using namespace rtv;
     RTV_REQUIRE(fcur && fprev && psi && mu && dchi && rho && row, "null device pointer");
     RTV_REQUIRE(n > 0 && m > 0 && nz > 0 && nzp > 0 && nz % nzp == 0, "invalid sizes (nz must be a multiple of nzp)");
-    RTV_REQUIRE(m <= 32, "at most 32 basis functions are supported");
     RTV_REQUIRE(((uintptr_t)psi % 16) == 0 && ((uintptr_t)mu % 16) == 0, "psi and mu must be 16-byte aligned");
     RTV_REQUIRE(std::isfinite(lam0) && std::isfinite(lam1) && lam1 != 0.f, "lambda1 must be finite and nonzero");
     const int nch = nz / nzp;
@@ -1666,10 +1680,12 @@ static void prox_noalloc(const float *fcur, const float *fprev, float *psi, floa
     hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, stream, red, rho, row, counter, 1, 0);
     RTV_HIP(hipGetLastError());
     const dim3 grid(Solver::tv_blocks(n, nch)), thr(TV_THREADS);
-    if (m <= 4) hipLaunchKernelGGL(k_divchi<4>, grid, thr, 0, stream, p4, m4, dchi, a);
-    else if (m <= 8) hipLaunchKernelGGL(k_divchi<8>, grid, thr, 0, stream, p4, m4, dchi, a);
-    else if (m <= 16) hipLaunchKernelGGL(k_divchi<16>, grid, thr, 0, stream, p4, m4, dchi, a);
-    else hipLaunchKernelGGL(k_divchi<32>, grid, thr, 0, stream, p4, m4, dchi, a);
+    if (m <= 4) hipLaunchKernelGGL(k_divchi<4>, grid, thr, 0, stream, p4, m4, dchi, a, 0);
+    else if (m <= 8) hipLaunchKernelGGL(k_divchi<8>, grid, thr, 0, stream, p4, m4, dchi, a, 0);
+    else if (m <= 16) hipLaunchKernelGGL(k_divchi<16>, grid, thr, 0, stream, p4, m4, dchi, a, 0);
+    else
+        for (int i0 = 0; i0 < m; i0 += 32)  // groups of 32 frames
+            hipLaunchKernelGGL(k_divchi<32>, grid, thr, 0, stream, p4, m4, dchi, a, i0);
     RTV_HIP(hipGetLastError());
 }
 

@@ -73,6 +73,14 @@ def _case(name):
         n, nt, m, nz = 40, 14, 5, 2
         th = B.angles(7, 2)
         phi, center = B.takephi(nt, m), n / 2 + 0.25
+    elif name == "n64_m40_pair_groups":   # 20 frame pairs: gather / scatter launches of 16 + 4 pairs
+        n, nt, m, nz = 64, 48, 40, 2
+        th = B.angles(24, 2)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
+    elif name == "n32_m65_pair_groups":   # 33 pairs: 16 + 16 + 1
+        n, nt, m, nz = 32, 80, 65, 1
+        th = np.sort(rng.uniform(0, 4 * np.pi, nt)).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2 - 0.5
     elif name == "n56_radix7_small":      # n = 56 = 8*7, 2n = 112 = 16*7
         n, nt, m, nz = 56, 15, 3, 2
         th = np.sort(rng.uniform(0, 3 * np.pi, nt)).astype(np.float32)
@@ -84,7 +92,7 @@ def _case(name):
 
 CASES = ["n64_rand", "n72_mixed_radix_m3", "n384_mixed_radix", "cfg_small", "n64_m20_two_blocks",
          "n1024_column_rounds", "n2048_column_rounds", "n480_radix5", "n504_radix7", "n40_radix5_small",
-         "n56_radix7_small"]
+         "n56_radix7_small", "n64_m40_pair_groups", "n32_m65_pair_groups"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -243,6 +251,7 @@ def _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed, basis="fourier", lam
     (64, 4, 4, 2, 32, 4, 4, 3),
     (48, 2, 1, 4, 12, 3, 3, 1),      # nzp = 1, odd m
     (32, 4, 2, 2, 16, 19, 3, 2),     # m = 19: two MFMA column blocks, odd m
+    (32, 4, 2, 2, 24, 41, 3, 2),     # m = 41: frame groups 32 + 9, pair groups 16 + 5
 ])
 def test_run_parity_small(gpu, n, nz, nzp, nrot, ntr, m, niter, titer):
     """Result, rho sequence, r, s, ||f(k+1)-f(k)||^2, ||f(k+1)||^2 and both objective
