@@ -39,6 +39,10 @@ constexpr int FFT_THREADS = 1024;
 constexpr int TS = 16;          // spectral tile edge (cells) for gather/scatter
 constexpr int KW = 9;           // Gaussian window taps per axis, 2K+1 with K = 4
 constexpr int TH = TS + KW - 1; // tile + halo rows loaded by the gather
+// Tiles per side of the 2n x 2n grid.  When 2n is not a multiple of TS the last
+// tile row / column is narrower: its cells past 2n wrap to the first tile, so
+// the gather reads them modulo 2n and the scatter writes only the cells < 2n.
+__host__ __device__ constexpr int tiles_per_side(int N) { return (N + TS - 1) / TS; }
 
 __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 
@@ -332,7 +336,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     __shared__ int sid[GM_ITEMS];
     __shared__ int sly[GM_ITEMS], slx[GM_ITEMS];
     const int tid = threadIdx.x;
-    const int N = 2 * n, nT = N / TS;
+    const int N = 2 * n, nT = tiles_per_side(N);
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int e = L % nentries;
     const int z0 = (L / nentries) * zg;
@@ -699,7 +703,7 @@ __global__ void __launch_bounds__(1024) k_scatter_weights(const AdjItem *__restr
     const int4 ch = achunk[ci];
     const int sm = threadIdx.x >> 3, part = threadIdx.x & 7;
     if (sm >= ch.z) return;
-    const int N = 2 * n, nT = N / TS;
+    const int N = 2 * n, nT = tiles_per_side(N);
     const int ty0 = (ch.x / nT) * TS, tx0 = (ch.x % nT) * TS;
     const AdjItem g = aitems[ch.y + sm];
     int dy = (g.rc & 0xffff) - ty0;
@@ -742,7 +746,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
     __shared__ float sP[ZW][NS][NCP];
     __shared__ unsigned char sl[SM_LB];
     const int tid = threadIdx.x;
-    const int N = 2 * n, nT = N / TS;
+    const int N = 2 * n, nT = tiles_per_side(N);
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int e = L % ntiles;
     const int z0 = (L / ntiles) * ZW;
@@ -878,10 +882,11 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
                 im[v] = acc[zz][1][v];
             }
         }
-        if (zz < zc && li < MP) {
+        // cells past 2n (a narrow last tile) are not this tile's; 2n and tx0 + bc0 are even
+        if (zz < zc && li < MP && ty0 + br0 + lk < N) {
             float4 *o = reinterpret_cast<float4 *>(Gt + ((size_t)(z0 + zz) * MPt + p0 + li) * N * N);
-            o[0] = make_float4(re[0], im[0], re[1], im[1]);
-            o[1] = make_float4(re[2], im[2], re[3], im[3]);
+            if (tx0 + bc0 < N) o[0] = make_float4(re[0], im[0], re[1], im[1]);
+            if (tx0 + bc0 + 2 < N) o[1] = make_float4(re[2], im[2], re[3], im[3]);
         }
     }
 }

@@ -105,7 +105,7 @@ static int usfft_K(int n, float mu) {
 #define RTV_GM_BALANCE 1  // gather block placement (build-time constant): 0 in order, 1 / 2 balanced
 #endif
 static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *theta, const float *phi, float center) {
-    const int N = 2 * n, nT = N / TS, K = 4;
+    const int N = 2 * n, nT = tiles_per_side(N), K = 4;
     const float mu = usfft_mu(n);
     if (usfft_K(n, mu) != K) throw Error(-1, "USFFT half-width differs from 4 for this n");
     const double mud = mu;
@@ -279,10 +279,16 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         for (size_t id = 0; id < ni; id++) {
             if (ftile[id] < 0) continue;
             const SampleGeom &g = hg.geo[id];
-            const int ty[2] = {g.row0 / TS, ((g.row0 + KW - 1) % N) / TS};
-            const int tx[2] = {g.col0 / TS, ((g.col0 + KW - 1) % N) / TS};
-            for (int a = 0; a < (ty[1] != ty[0] ? 2 : 1); a++)
-                for (int b = 0; b < (tx[1] != tx[0] ? 2 : 1); b++) {
+            // the tiles the 9 x 9 window meets: 2 per axis at most, 3 when a narrow
+            // last tile (2n not a multiple of 16) lies inside the window
+            int ty[3], tx[3], nty = 0, ntx = 0;
+            for (int a = 0; a < KW; a++) {
+                const int r = ((g.row0 + a) % N) / TS, c = ((g.col0 + a) % N) / TS;
+                if (nty == 0 || ty[nty - 1] != r) ty[nty++] = r;
+                if (ntx == 0 || tx[ntx - 1] != c) tx[ntx++] = c;
+            }
+            for (int a = 0; a < nty; a++)
+                for (int b = 0; b < ntx; b++) {
                     int dy = (g.row0 - ty[a] * TS + N) % N, dx = (g.col0 - tx[b] * TS + N) % N;
                     const int ly0 = dy < TS ? dy : dy - N, lx0 = dx < TS ? dx : dx - N;
                     const unsigned loc = (unsigned)((ly0 + 64) << 8 | (lx0 + 64));
@@ -437,8 +443,8 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
         for (const TileEntry &te : hg.atiles) {
             const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
             bool meets = te.count > 0;
-            for (int r = ty0; r < ty0 + TS && !meets; r++)
-                for (int c = tx0; c < tx0 + TS && !meets; c++) meets = in_run_h(r, c);
+            for (int r = ty0; r < std::min(ty0 + TS, N) && !meets; r++)
+                for (int c = tx0; c < std::min(tx0 + TS, N) && !meets; c++) meets = in_run_h(r, c);
             if (meets) act.push_back(te);
         }
         hg.atiles.swap(act);
@@ -618,7 +624,7 @@ struct Solver {
     Solver(int n_, int nt_, int m_, int nz_, int nzp_, int ngpus_) : n(n_), nt(nt_), M(m_), nz(nz_), nzp(nzp_) {
         RTV_REQUIRE(n > 0 && nt > 0 && M > 0 && nz > 0 && nzp > 0, "all sizes must be positive");
         RTV_REQUIRE(nz % nzp == 0, "nz must be a multiple of nzp (the reference silently drops the remainder)");
-        RTV_REQUIRE(n % 8 == 0, "n must be a multiple of 8 (16-cell spectral tiles of the 2n grid)");
+        RTV_REQUIRE(n % 2 == 0 && n >= 16, "n must be even and at least 16");
         nch = nz / nzp;
         MP = (M + 1) / 2;
         ngpus = std::max(1, std::min(ngpus_, nch));
@@ -638,7 +644,7 @@ struct Solver {
         const size_t V4 = nch * (nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1);
         const size_t state = 3 * V * 4 + 2 * V4 * 16 + V * 4 + (size_t)nz * nt * n * 4;
         const size_t per_slice = MP * n * N * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 24 + (size_t)nt * n * 4;
-        const size_t ni = 2 * (size_t)nt * n, ntile = (N / TS) * (N / TS);
+        const size_t ni = 2 * (size_t)nt * n, ntile = (size_t)tiles_per_side(N) * tiles_per_side(N);
         // geo 16 B, gather slots <= 16 ints, adjoint lists <= 4 records of 16 + 8 B and
         // block lists <= 9 bytes per item (16 B spare); per adjoint chunk its range, 16
         // block ranges, up to 3 padding bytes per block, its record range and SM_CH weight

@@ -81,6 +81,18 @@ def _case(name):
         n, nt, m, nz = 32, 80, 65, 1
         th = np.sort(rng.uniform(0, 4 * np.pi, nt)).astype(np.float32)
         phi, center = B.takephi(nt, m), n / 2 - 0.5
+    elif name == "n36_narrow_tile":       # 2n = 72 = 4*16 + 8: a narrow last tile row / column
+        n, nt, m, nz = 36, 20, 4, 2
+        th = np.sort(rng.uniform(0, 4 * np.pi, nt)).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
+    elif name == "n50_three_tiles":       # 2n = 100 = 6*16 + 4: windows meeting three tiles per axis
+        n, nt, m, nz = 50, 24, 6, 1
+        th = B.angles(12, 2)
+        phi, center = B.takephi(nt, m), n / 2 - 0.75
+    elif name == "n100_narrow_tile":      # 2n = 200 = 12*16 + 8, m = 18: two column blocks
+        n, nt, m, nz = 100, 36, 18, 1
+        th = np.sort(rng.uniform(0, 6 * np.pi, nt)).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2
     elif name == "n56_radix7_small":      # n = 56 = 8*7, 2n = 112 = 16*7
         n, nt, m, nz = 56, 15, 3, 2
         th = np.sort(rng.uniform(0, 3 * np.pi, nt)).astype(np.float32)
@@ -92,7 +104,8 @@ def _case(name):
 
 CASES = ["n64_rand", "n72_mixed_radix_m3", "n384_mixed_radix", "cfg_small", "n64_m20_two_blocks",
          "n1024_column_rounds", "n2048_column_rounds", "n480_radix5", "n504_radix7", "n40_radix5_small",
-         "n56_radix7_small", "n64_m40_pair_groups", "n32_m65_pair_groups"]
+         "n56_radix7_small", "n64_m40_pair_groups", "n32_m65_pair_groups", "n36_narrow_tile",
+         "n50_three_tiles", "n100_narrow_tile"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -252,6 +265,7 @@ def _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed, basis="fourier", lam
     (48, 2, 1, 4, 12, 3, 3, 1),      # nzp = 1, odd m
     (32, 4, 2, 2, 16, 19, 3, 2),     # m = 19: two MFMA column blocks, odd m
     (32, 4, 2, 2, 24, 41, 3, 2),     # m = 41: frame groups 32 + 9, pair groups 16 + 5
+    (50, 4, 2, 2, 16, 4, 3, 2),      # n = 50: 2n = 100, narrow last tiles
 ])
 def test_run_parity_small(gpu, n, nz, nzp, nrot, ntr, m, niter, titer):
     """Result, rho sequence, r, s, ||f(k+1)-f(k)||^2, ||f(k+1)||^2 and both objective
