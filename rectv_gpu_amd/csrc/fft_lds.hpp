@@ -3,7 +3,8 @@
 // T transforms of length N live in LDS at buf[t*stride + i].  Each stage reads
 // every butterfly's R inputs into registers, synchronises, applies the radix-R
 // DFT and writes the autosorted outputs back into the same buffer, so one
-// buffer of T*stride complex values is enough.  Radices 2,3,4,5,7,8; twiddles
+// buffer of T*stride complex values is enough.  Radices 2,3,4,5,7,8 (and any
+// larger prime through a direct O(R) per output stage); twiddles
 // come from a table tw[m] = exp(-2 pi i m / N) computed in FP64 on the host.
 // Forward = exp(-2 pi i jk/N), inverse = exp(+2 pi i jk/N), both unnormalised.
 // Requirement: blockDim.x >= T * ceil(N / E) (E = max elements held per thread).
@@ -13,6 +14,7 @@
 namespace rtv {
 
 constexpr int FFT_MAX_STAGES = 16;
+constexpr int FFT_MAX_PRIME = 127;  // largest prime factor of a transform length
 struct FftPlan {
     int n;
     int nst;
@@ -227,6 +229,49 @@ __device__ void fft_stage(float2 *buf, int stride, int T, int N, int Ns, int lgN
     __syncthreads();
 }
 
+// A stage of any radix R (the prime factors above 7, which have no butterfly of
+// their own): every thread forms its E outputs of the stage directly,
+//   y[jq Ns R + k + q Ns] = sum_r x[j + r N/R] W_N^(r (k N/(Ns R) + q N/R)),  j = jq Ns + k,
+// O(R) LDS reads and table twiddles per output, then all outputs are written back.
+template <int E, bool INV>
+__device__ void fft_stage_gen(float2 *buf, int stride, int T, int N, int Ns, int R, const float2 *__restrict__ tw,
+                              int t, int jl, int tpt) {
+    const int nb = N / R;
+    const int twstep = N / (Ns * R);
+    const bool active = t < T;
+    float2 *row = buf + t * stride;
+    float2 y[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int d = jl + e * tpt;
+        y[e] = make_float2(0.f, 0.f);
+        if (active && d < N) {
+            const int k = d % Ns, q = (d / Ns) % R, jq = d / (Ns * R);
+            const int j = jq * Ns + k;
+            const int step = (k * twstep + q * nb) % N;
+            int m = 0;
+            float2 acc = make_float2(0.f, 0.f);
+            for (int r = 0; r < R; r++) {
+                float2 w = tw[m];
+                if (INV) w.y = -w.y;
+                const float2 x = row[lp(j + r * nb)];
+                acc.x += x.x * w.x - x.y * w.y;
+                acc.y += x.x * w.y + x.y * w.x;
+                m += step;
+                if (m >= N) m -= N;
+            }
+            y[e] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int d = jl + e * tpt;
+        if (active && d < N) row[lp(d)] = y[e];
+    }
+    __syncthreads();
+}
+
 // Threads per transform for length N (each thread holds at most E elements).
 __host__ __device__ inline int fft_tpt(int N, int E) { return (N + E - 1) / E; }
 
@@ -234,8 +279,9 @@ __host__ __device__ inline int fft_tpt(int N, int E) { return (N + E - 1) / E; }
 // blockDim.x >= T * fft_tpt(p.n, E).  Caller must __syncthreads() before.
 // ODD selects the instantiation that also carries radices 3, 5, 7 (needed only
 // for lengths with those factors; the power-of-two variant keeps register use
-// low enough to avoid spills).
-template <int E, bool INV, bool ODD>
+// low enough to avoid spills); GEN adds the direct stage for primes above 7
+// (its own kernel variant, so that the common lengths do not pay for it).
+template <int E, bool INV, bool ODD, bool GEN = false>
 __device__ void fft_lds(float2 *buf, int stride, int T, const FftPlan &p, const float2 *__restrict__ twall,
                         int tid, int nthr) {
     (void)nthr;
@@ -255,6 +301,7 @@ __device__ void fft_lds(float2 *buf, int stride, int T, const FftPlan &p, const 
                     if (R == 3) fft_stage<3, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt);
                     else if (R == 5) fft_stage<5, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt);
                     else if (R == 7) fft_stage<7, E, INV>(buf, stride, T, p.n, Ns, lg, tw, t, jl, tpt);
+                    else if constexpr (GEN) fft_stage_gen<E, INV>(buf, stride, T, p.n, Ns, R, tw, t, jl, tpt);
                 }
                 break;
         }

@@ -91,7 +91,8 @@ constexpr int RAY_WG = RTV_RAY_WG;  // threads of the per-ray transform workgrou
 // FFT_WG threads, which lets it hold a stage's butterflies in registers without
 // spilling (up to 256 VGPRs); NP = -1 is the same path for transforms that need
 // more than FFT_WG threads (length > 16 FFT_WG).
-__host__ __device__ constexpr int fft_bound(int NP) { return NP == 0 ? FFT_WG : FFT_THREADS; }
+__host__ __device__ constexpr int fft_bound(int NP) { return NP == 0 || NP == -2 ? FFT_WG : FFT_THREADS; }
+// NP = -2: runtime plan with a prime factor above 7 (direct stages), at most FFT_WG threads
 __host__ __device__ constexpr int fft_rows_p2(int L, int wg = FFT_WG) {
     return (wg / (L / FFT_E)) < 1 ? 1 : ((wg / (L / FFT_E)) < 16 ? (wg / (L / FFT_E)) : 16);
 }
@@ -211,7 +212,7 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__r
             lds[pt * stride + lp(q)] = make_float2(0.f, 0.f);
         }
         __syncthreads();
-        fft_lds<FFT_E, false, true>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_lds<FFT_E, false, true, NP == -2>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * N; idx += nthr) {
             const int pt = idx / N, kx = idx - pt * N;
             H[h_index((size_t)z * MP + p0 + pt, n, y, kx, HG)] = cscale(lds[pt * stride + lp(kx)], sgn(kx));
@@ -265,7 +266,7 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
             lds[t * stride + lp(q)] = make_float2(0.f, 0.f);
         }
         __syncthreads();
-        fft_lds<FFT_E, false, true>(lds, stride, T, plan, tw, tid, nthr);
+        fft_lds<FFT_E, false, true, NP == -2>(lds, stride, T, plan, tw, tid, nthr);
         float2 *Gs = G + (size_t)zj * N * N + (size_t)kx0 * N;
         for (int idx = tid; idx < T * N; idx += nthr) {
             const int t = idx / N, ky = idx - t * N;
@@ -625,7 +626,7 @@ k_ray(const float2 *__restrict__ gq, float2 *__restrict__ q, const float *__rest
             lds[t * stride + lp(s)] = v;
         }
         __syncthreads();
-        fft_lds<FFT_E, true, true>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_lds<FFT_E, true, true, NP == -2>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
             const size_t o = (size_t)(r0 + t) * n + s;
@@ -643,7 +644,7 @@ k_ray(const float2 *__restrict__ gq, float2 *__restrict__ q, const float *__rest
     }
     if (mode & RAY_ADJ) {
         __syncthreads();
-        fft_lds<FFT_E, false, true>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_lds<FFT_E, false, true, NP == -2>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, s = idx - t * n;
             float2 v = make_float2(0.f, 0.f);
@@ -939,7 +940,7 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
                 in_run(runs[kx0 + t], ky, N) ? cscale(Gs[(size_t)t * N + ky], sgn(ky)) : make_float2(0.f, 0.f);
         }
         __syncthreads();
-        fft_lds<FFT_E, true, true>(lds, stride, T, plan, tw, tid, nthr);
+        fft_lds<FFT_E, true, true, NP == -2>(lds, stride, T, plan, tw, tid, nthr);
         const int ga = h_group_adj(N);
         for (int idx = tid; idx < T * n; idx += nthr) {
             const int t = idx % T, y = idx / T;
@@ -1013,7 +1014,7 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
             lds[t * stride + lp(kx)] = cscale(H[h_index(zj, n, y0 + t, kx, ga)], sgn(kx));
         }
         __syncthreads();
-        fft_lds<FFT_E, true, true>(lds, stride, Tc, plan, tw, tid, nthr);
+        fft_lds<FFT_E, true, true, NP == -2>(lds, stride, Tc, plan, tw, tid, nthr);
         float2 *vs = v + ((size_t)zj * n + y0) * n;
         for (int idx = tid; idx < Tc * n; idx += nthr) {
             const int t = idx / n, x = idx - t * n;

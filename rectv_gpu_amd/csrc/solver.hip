@@ -24,10 +24,18 @@ thread_local std::string g_last_error;  // message of the last failing C-ABI cal
 static const double PI_REF = 3.1415926535;  // kernels_radonusfft.cuh:1
 static const double PI_EXACT = 3.14159265358979323846;
 
+// a prime factor above 7 (planned as a direct stage)
+static bool has_big_prime(int len) {
+    for (int R : {2, 3, 5, 7})
+        while (len % R == 0) len /= R;
+    return len > 1;
+}
+
 // ---------------------------------------------------------------- kernel pickers
 // FFT kernels are instantiated for power-of-two lengths (compile-time index
 // math) and with a runtime plan for every other length: NP = 0 (at most FFT_WG
-// threads per workgroup) or NP = -1 (longer transforms, up to FFT_THREADS).
+// threads per workgroup) or NP = -1 (longer transforms, up to FFT_THREADS), and
+// NP = -2 for lengths with a prime factor above 7 (at most FFT_WG threads).
 #define RTV_PICKER(NAME, K)                                                                        \
     static decltype(&K<0>) NAME(int len) {                                                         \
         switch (len) {                                                                             \
@@ -38,7 +46,7 @@ static const double PI_EXACT = 3.14159265358979323846;
             case 1024: return K<1024>;                                                             \
             case 2048: return K<2048>;                                                             \
             case 4096: return K<4096>;                                                             \
-            default: return fft_tpt(len, FFT_E) > FFT_WG ? K<-1> : K<0>;                          \
+            default: return has_big_prime(len) ? K<-2> : fft_tpt(len, FFT_E) > FFT_WG ? K<-1> : K<0>;   \
         }                                                                                          \
     }
 RTV_PICKER(pick_fwd_rows, k_fwd_rows)
@@ -46,6 +54,7 @@ RTV_PICKER(pick_fwd_cols, k_fwd_cols)
 RTV_PICKER(pick_adj_cols, k_adj_cols)
 RTV_PICKER(pick_adj_rows, k_adj_rows)
 RTV_PICKER(pick_ray, k_ray)
+#undef RTV_PICKER
 
 // ---------------------------------------------------------------- FFT plans
 static bool make_plan(int N, FftPlan &p) {
@@ -59,6 +68,12 @@ static bool make_plan(int N, FftPlan &p) {
             r /= R;
         }
     }
+    // remaining prime factors up to FFT_MAX_PRIME: direct stages
+    for (int R = 11; R <= FFT_MAX_PRIME && r > 1; R += 2)
+        while (r % R == 0 && p.nst < FFT_MAX_STAGES) {
+            p.radix[p.nst++] = R;
+            r /= R;
+        }
     return r == 1;
 }
 
@@ -662,7 +677,7 @@ struct Solver {
     void init_device() {
         N = 2 * n;
         RTV_REQUIRE(make_plan(n, plan_n) && make_plan(N, plan_N),
-                    "n must factor into radices 2, 3, 5, 7 (FFT sizes n and 2n)");
+                    "n must have no prime factor above 127 (FFT sizes n and 2n)");
         odd_n = (n & (n - 1)) != 0;
         odd_N = (N & (N - 1)) != 0;
         int ndev = 0;
@@ -741,6 +756,8 @@ struct Solver {
             return std::max(1, std::min(std::max(1, FFT_WG / fft_tpt(len, FFT_E)), lds_max / (8 * stride)));
         };
         RTV_REQUIRE(fft_tpt(N, FFT_E) <= FFT_THREADS, "2n-point transform needs too many threads");
+        RTV_REQUIRE(!has_big_prime(N) || fft_tpt(N, FFT_E) <= FFT_WG,
+                    "2n-point transforms with a prime factor above 7 are limited to 2n <= 8192");
         const int tN = tmax(N, st_N), tn = tmax(n, st_n);
         RTV_REQUIRE(tN >= 1 && 8 * st_N <= lds_max, "2n-point transform does not fit in LDS");
         T_cols = 1;

@@ -93,6 +93,18 @@ def _case(name):
         n, nt, m, nz = 100, 36, 18, 1
         th = np.sort(rng.uniform(0, 6 * np.pi, nt)).astype(np.float32)
         phi, center = B.takephi(nt, m), n / 2
+    elif name == "n44_radix11":           # n = 44 = 4*11, 2n = 88 = 8*11: direct prime stage
+        n, nt, m, nz = 44, 18, 4, 1
+        th = B.angles(9, 2)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
+    elif name == "n34_radix17":           # n = 34 = 2*17, 2n = 68 = 4*17 (a 2448-pixel detector has 17)
+        n, nt, m, nz = 34, 16, 3, 2
+        th = np.sort(rng.uniform(0, 2 * np.pi, nt)).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2 - 0.5
+    elif name == "n136_radix17":          # n = 136 = 8*17, 2n = 272 = 16*17
+        n, nt, m, nz = 136, 12, 2, 1
+        th = B.angles(6, 2)
+        phi, center = B.takephi(nt, m), n / 2
     elif name == "n56_radix7_small":      # n = 56 = 8*7, 2n = 112 = 16*7
         n, nt, m, nz = 56, 15, 3, 2
         th = np.sort(rng.uniform(0, 3 * np.pi, nt)).astype(np.float32)
@@ -105,7 +117,7 @@ def _case(name):
 CASES = ["n64_rand", "n72_mixed_radix_m3", "n384_mixed_radix", "cfg_small", "n64_m20_two_blocks",
          "n1024_column_rounds", "n2048_column_rounds", "n480_radix5", "n504_radix7", "n40_radix5_small",
          "n56_radix7_small", "n64_m40_pair_groups", "n32_m65_pair_groups", "n36_narrow_tile",
-         "n50_three_tiles", "n100_narrow_tile"]
+         "n50_three_tiles", "n100_narrow_tile", "n44_radix11", "n34_radix17", "n136_radix17"]
 
 
 @pytest.mark.parametrize("name", CASES)
