@@ -1387,6 +1387,11 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
 // 1/2 ||R f - g||^2 of the entering iterate (NaN when no inner step ran),
 // sum |grad fn|}; *iter is then incremented.
 constexpr int HIST_W = 7;
+// dst[i] += src[i], i < cnt (norms of the out-of-core windows)
+__global__ void k_accumulate(double *__restrict__ dst, const double *__restrict__ src, int cnt) {
+    const int i = threadIdx.x;
+    if (i < cnt) dst[i] += src[i];
+}
 __global__ void k_rho_rule(const double *__restrict__ red, float *__restrict__ rho, double *__restrict__ hist,
                            int *__restrict__ iter, int cap, int has_data) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;

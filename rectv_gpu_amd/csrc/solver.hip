@@ -587,6 +587,26 @@ struct Solver {
     DBuf<float4> d_psi, d_mu;
     DBuf<float> d_dchi;  // div(mu - rho psi) for the current outer iteration
     DBuf<float> d_g;
+    // Views of psi, mu, dchi and the data used by the ADMM kernels, which address
+    // them by global chunk / slice index, and the chunk range [c_lo, c_lo + c_n)
+    // they run over: the device arrays and all chunks when the state is resident;
+    // out of core, the device window arrays offset back by the window's first
+    // chunk (never dereferenced outside the window) and the window's chunks.
+    float4 *v_psi = nullptr, *v_mu = nullptr;
+    float *v_dchi = nullptr;
+    const float *v_g = nullptr;
+    int c_lo = 0, c_n = 0;
+    // Out of core (volumes whose state does not fit in device memory, rectv.cu:12-16,
+    // 143-197): f, fn, psi, mu and the data stay in pinned host memory and each outer
+    // iteration streams windows of W chunks (plus one halo slice each side) through
+    // the device; the norms of all windows are summed before the rho rule.
+    bool ooc = false;
+    int W = 0;
+    float *h_f[2] = {nullptr, nullptr};  // h_f[hf]: current iterate, h_f[hf ^ 1]: the next
+    int hf = 0;
+    float4 *h_psi = nullptr, *h_mu = nullptr;
+    float *h_g = nullptr;
+    DBuf<double> d_acc;  // norms summed over the windows
     // penalty rho, its rule and the per-iteration norms live on the device
     // (k_rho_rule), so an outer iteration needs no host round trip
     DBuf<float> d_rho;
@@ -691,22 +711,55 @@ struct Solver {
         lds_max -= 1024;  // room for the kernels' static shared arrays
         RTV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         setup_fft();
-        // persistent state
-        const size_t V = (size_t)nz * M * n * n;
-        for (auto &b : d_f) b.alloc(V, bytes);
-        const size_t V4 = (size_t)nch * (nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1);
-        d_psi.alloc(V4, bytes);
-        d_mu.alloc(V4, bytes);
-        d_dchi.alloc(V, bytes);
-        d_g.alloc((size_t)nz * nt * n, bytes);
-        // scratch plan
+        // persistent state: resident, or out of core in windows of W chunks when asked
+        // (RECTV_STREAM_CHUNKS = chunks per window) or when the resident state and one
+        // chunk of operator scratch would not fit in free device memory
+        const size_t S = (size_t)M * n * n, D = (size_t)(nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1);
+        const size_t V = (size_t)nz * S, V4 = (size_t)nch * D;
         const size_t per_slice = (size_t)MP * n * N * 8 + (size_t)MP * N * N * 8 + (size_t)MP * n * n * 8 +
                                  (size_t)nt * n * 24 + (size_t)nt * n * 4;
+        const size_t per_chunk = 4 * (size_t)nzp * S * 4 + 2 * D * 16 + (size_t)nzp * nt * n * 4;
         size_t freeb = 0, totb = 0;
+        RTV_HIP(hipMemGetInfo(&freeb, &totb));
+        W = nch;
+        if (const char *e = std::getenv("RECTV_STREAM_CHUNKS")) {
+            if (std::atoi(e) > 0) W = std::min(nch, std::atoi(e));
+        } else if ((double)per_chunk * nch + (double)per_slice * nzp > 0.9 * (double)freeb) {
+            W = (int)std::max(1.0, std::min((double)nch, std::floor(0.55 * (double)freeb /
+                                                                     ((double)per_chunk + (double)per_slice * nzp))));
+        }
+        ooc = W < nch;
+        const size_t ws = ooc ? (size_t)W * nzp + 2 : (size_t)nz;  // f slices on the device (window + 2 halos)
+        for (auto &b : d_f) b.alloc(ws * S, bytes);
+        d_psi.alloc(ooc ? W * D : V4, bytes);
+        d_mu.alloc(ooc ? W * D : V4, bytes);
+        d_dchi.alloc(ooc ? (size_t)W * nzp * S : V, bytes);
+        d_g.alloc((ooc ? (size_t)W * nzp : (size_t)nz) * nt * n, bytes);
+        if (ooc) {
+            auto host = [&](void **p, size_t b) {
+                if (hipHostMalloc(p, b, hipHostMallocDefault) != hipSuccess || !*p) {
+                    *p = nullptr;
+                    throw Error(-3, "pinned host allocation of " + std::to_string(b) +
+                                        " bytes for the out-of-core state failed");
+                }
+            };
+            for (auto &h : h_f) host((void **)&h, V * sizeof(float));
+            host((void **)&h_psi, V4 * sizeof(float4));
+            host((void **)&h_mu, V4 * sizeof(float4));
+            host((void **)&h_g, (size_t)nz * nt * n * sizeof(float));
+            d_acc.alloc(8, bytes);
+        }
+        v_psi = d_psi.p;
+        v_mu = d_mu.p;
+        v_dchi = d_dchi.p;
+        v_g = d_g.p;
+        c_lo = 0;
+        c_n = nch;
+        // scratch plan
         RTV_HIP(hipMemGetInfo(&freeb, &totb));
         double budget = 0.45 * (double)freeb;
         if (const char *e = std::getenv("RECTV_SCRATCH_MB")) budget = std::atof(e) * 1048576.0;
-        cb = (int)std::max(1.0, std::min((double)nch, std::floor(budget / ((double)per_slice * nzp))));
+        cb = (int)std::max(1.0, std::min((double)W, std::floor(budget / ((double)per_slice * nzp))));
         zb = cb * nzp;
         d_H.alloc((size_t)zb * MP * n * N, bytes);
         d_G.alloc((size_t)zb * MP * N * N, bytes);
@@ -726,8 +779,9 @@ struct Solver {
         if (std::getenv("RECTV_VERBOSE"))
             fprintf(stderr,
                     "rectv: n=%d ntheta=%d m=%d nz=%d nzp=%d lds_max=%d T_rows=%d T_cols=%d/%d T_ray=%d T_arows=%d "
-                    "chunks/batch=%d device bytes=%zu\n",
-                    n, nt, M, nz, nzp, lds_max, T_rows, T_fcols, T_acols, T_ray, T_arows, cb, bytes);
+                    "chunks/batch=%d device bytes=%zu%s\n",
+                    n, nt, M, nz, nzp, lds_max, T_rows, T_fcols, T_acols, T_ray, T_arows, cb, bytes,
+                    ooc ? (" (out of core: " + std::to_string(W) + " chunks per window)").c_str() : "");
     }
 
     int fft_threads(int T, int len) const {
@@ -872,14 +926,21 @@ struct Solver {
                 hipLaunchKernelGGL(k_scatter_weights, dim3(nchunks), dim3(1024), 0, st, d_aitems.p, d_achunk.p,
                                    d_wtab.p, n, beta);
             });
-        RTV_HIP(hipMemcpyAsync(d_g.p, g, (size_t)nz * nt * n * sizeof(float), hipMemcpyHostToDevice, st));
+        if (ooc) std::memcpy(h_g, g, (size_t)nz * nt * n * sizeof(float));
+        else RTV_HIP(hipMemcpyAsync(d_g.p, g, (size_t)nz * nt * n * sizeof(float), hipMemcpyHostToDevice, st));
         RTV_HIP(hipStreamSynchronize(st));
         inputs = true;
     }
 
     void reset() {
         const size_t V = (size_t)nz * M * n * n;
-        for (auto &b : d_f) RTV_HIP(hipMemsetAsync(b.p, 0, V * sizeof(float), st));
+        if (ooc) {
+            std::memset(h_f[0], 0, V * sizeof(float));
+            std::memset(h_psi, 0, (size_t)nch * dual_chunk() * sizeof(float4));
+            std::memset(h_mu, 0, (size_t)nch * dual_chunk() * sizeof(float4));
+            hf = 0;
+        }
+        for (auto &b : d_f) RTV_HIP(hipMemsetAsync(b.p, 0, b.n * sizeof(float), st));
         RTV_HIP(hipMemsetAsync(d_psi.p, 0, d_psi.n * sizeof(float4), st));
         RTV_HIP(hipMemsetAsync(d_mu.p, 0, d_mu.n * sizeof(float4), st));
         RTV_HIP(hipMemsetAsync(d_dchi.p, 0, d_dchi.n * sizeof(float), st));  // psi = mu = 0
@@ -1039,7 +1100,7 @@ struct Solver {
         const dim3 grid(tv_blocks(n, ncb));
         for (int i0 = 0; i0 < M; i0 += MM)  // groups of MM frames (one group unless M > 32)
             launch("update", [&] {
-                hipLaunchKernelGGL(k_update<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, nxt, d_dchi.p, d_v.p, ta, zv0,
+                hipLaunchKernelGGL(k_update<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, nxt, v_dchi, d_v.p, ta, zv0,
                                    i0);
             });
     }
@@ -1049,24 +1110,35 @@ struct Solver {
         else if (M <= 16) launch_update<16>(cur, A, nxt, ta, ncb, zv0);
         else launch_update<32>(cur, A, nxt, ta, ncb, zv0);
     }
-    void prox(const float *cur, const float *A, const TvArgs &ta, bool has_data) {
-        const dim3 grid(((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * nch);
+    // prox over chunks [c_lo, c_lo + c_n), its norm partials reduced into d_red[0..4]
+    void prox_partial(const float *cur, const float *A, const TvArgs &ta) {
+        const dim3 grid(((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * c_n);
+        TvArgs tb = ta;
+        tb.chunk0 = c_lo;
         launch("prox", [&] {
-            hipLaunchKernelGGL(k_prox, grid, dim3(64 * PX_WAVES), 0, st, cur, A, d_psi.p, d_mu.p, ta, d_part.p);
+            hipLaunchKernelGGL(k_prox, grid, dim3(64 * PX_WAVES), 0, st, cur, A, v_psi, v_mu, tb, d_part.p);
         });
         const int cnt = grid.x;
         launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, PROX_W); });
+    }
+    void rho_rule(bool has_data) {
         launch("rho", [&] {
             hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, st, d_red.p, d_rho.p, d_hist.p, d_iter.p, HIST_CAP,
                                has_data ? 1 : 0);
         });
     }
+    void prox(const float *cur, const float *A, const TvArgs &ta, bool has_data) {
+        prox_partial(cur, A, ta);
+        rho_rule(has_data);
+    }
     template <int MM>
     void launch_divchi(const TvArgs &ta) {
-        const dim3 grid(tv_blocks(n, nch));
+        const dim3 grid(tv_blocks(n, c_n));
+        TvArgs tb = ta;
+        tb.chunk0 = c_lo;
         for (int i0 = 0; i0 < M; i0 += MM)
             launch("divchi", [&] {
-                hipLaunchKernelGGL(k_divchi<MM>, grid, dim3(TV_THREADS), 0, st, d_psi.p, d_mu.p, d_dchi.p, ta, i0);
+                hipLaunchKernelGGL(k_divchi<MM>, grid, dim3(TV_THREADS), 0, st, v_psi, v_mu, v_dchi, tb, i0);
             });
     }
     void divchi(const TvArgs &ta) {
@@ -1106,16 +1178,86 @@ struct Solver {
         ia = cur;
     }
 
+    // Out of core: one outer iteration window by window.  A window of w chunks
+    // holds its f slices in device slots 1 .. w nzp of the three f buffers, the
+    // halo slices of the neighbouring chunks in slots 0 and w nzp + 1 of the first
+    // (the previous iterate A), and its psi, mu, dchi and data.  Per window:
+    // dchi = div(mu - rho psi) with the current rho, the titer inner steps, the
+    // prox (norm partials added into d_acc), then fn, psi and mu go back to the
+    // host.  After the last window the rho rule sees the norms of all chunks
+    // (rectv.cu:216-237).  Chunks read only f of other chunks, so windows are
+    // independent within an iteration, as the reference's chunk loop is.
+    template <typename T>
+    static T *shift_back(T *p, size_t elems) {  // p - elems, as an address (never dereferenced below p)
+        return reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) - elems * sizeof(T));
+    }
+    void one_iteration_ooc(int titer, float lam0, float lam1) {
+        const size_t S = (size_t)M * n * n, D = dual_chunk(), G1 = (size_t)nt * n;
+        const TvArgs ta = tv_args(lam1, lam0);
+        const float *hin = h_f[hf];
+        float *hout = h_f[hf ^ 1];
+        RTV_HIP(hipMemsetAsync(d_acc.p, 0, 8 * sizeof(double), st));
+        for (int g0 = 0; g0 < nch; g0 += W) {
+            const int w = std::min(W, nch - g0);
+            const size_t z0 = (size_t)g0 * nzp, zw = (size_t)w * nzp;
+            const size_t lo = g0 > 0 ? z0 - 1 : z0, hi = g0 + w < nch ? z0 + zw + 1 : z0 + zw;
+            RTV_HIP(hipMemcpyAsync(d_f[0].p + (lo + 1 - z0) * S, hin + lo * S, (hi - lo) * S * sizeof(float),
+                                   hipMemcpyHostToDevice, st));
+            RTV_HIP(hipMemcpyAsync(d_psi.p, h_psi + g0 * D, w * D * sizeof(float4), hipMemcpyHostToDevice, st));
+            RTV_HIP(hipMemcpyAsync(d_mu.p, h_mu + g0 * D, w * D * sizeof(float4), hipMemcpyHostToDevice, st));
+            RTV_HIP(hipMemcpyAsync(d_g.p, h_g + z0 * G1, zw * G1 * sizeof(float), hipMemcpyHostToDevice, st));
+            // views addressed by global chunk / slice index
+            float *fv[3];
+            for (int b = 0; b < 3; b++) fv[b] = shift_back(d_f[b].p + S, z0 * S);
+            v_psi = shift_back(d_psi.p, g0 * D);
+            v_mu = shift_back(d_mu.p, g0 * D);
+            v_dchi = shift_back(d_dchi.p, z0 * S);
+            v_g = shift_back(d_g.p, z0 * G1);
+            c_lo = g0;
+            c_n = w;
+            divchi(ta);
+            int cur = 0;
+            for (int k = 0; k < titer; k++) {
+                const int nxt = (k == 0) ? 1 : (cur == 1 ? 2 : 1);
+                inner_step(fv[cur], fv[0], fv[nxt], ta, k == 0);
+                if (k == 0)
+                    launch("accumulate", [&] {
+                        hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, st, d_acc.p + 5, d_red.p + 5, 1);
+                    });
+                cur = nxt;
+            }
+            prox_partial(fv[cur], fv[0], ta);
+            launch("accumulate", [&] {
+                hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, st, d_acc.p, d_red.p, PROX_W);
+            });
+            RTV_HIP(hipMemcpyAsync(hout + z0 * S, d_f[cur].p + S, zw * S * sizeof(float), hipMemcpyDeviceToHost, st));
+            RTV_HIP(hipMemcpyAsync(h_psi + g0 * D, d_psi.p, w * D * sizeof(float4), hipMemcpyDeviceToHost, st));
+            RTV_HIP(hipMemcpyAsync(h_mu + g0 * D, d_mu.p, w * D * sizeof(float4), hipMemcpyDeviceToHost, st));
+        }
+        v_psi = d_psi.p;
+        v_mu = d_mu.p;
+        v_dchi = d_dchi.p;
+        v_g = d_g.p;
+        c_lo = 0;
+        c_n = nch;
+        RTV_HIP(hipMemcpyAsync(d_red.p, d_acc.p, 8 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        rho_rule(titer > 0);
+        hf ^= 1;
+        // the host arrays are read and written by the copies above: finish them before
+        // the caller can read the result or the next iteration overwrites hout
+        RTV_HIP(hipStreamSynchronize(st));
+    }
+
     // One inner ADMM step (admm.cu:85-97) over all chunks: nxt = cur - 1/2 div(rho
     // grad cur + chi) - 1/2 R^*(R cur - g), halo slabs from A.  With data_term the
     // ray passes also leave the partial sums of ||R cur - g||^2, reduced into d_red[5].
     void inner_step(const float *cur, const float *A, float *nxt, const TvArgs &ta, bool data_term) {
         const size_t S = (size_t)M * n * n;
         size_t roff = 0;
-        for (int c0 = 0; c0 < nch; c0 += cb) {
-            const int ncb = std::min(cb, nch - c0);
+        for (int c0 = c_lo; c0 < c_lo + c_n; c0 += cb) {
+            const int ncb = std::min(cb, c_lo + c_n - c0);
             const int z0 = c0 * nzp, zc = ncb * nzp;
-            forward_to_rays(cur + z0 * S, zc, d_g.p + (size_t)z0 * nt * n, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ,
+            forward_to_rays(cur + z0 * S, zc, v_g + (size_t)z0 * nt * n, nullptr, RAY_FWD | RAY_SUBG | RAY_ADJ,
                             data_term ? d_rpart.p + roff : nullptr);
             roff += ray_blocks(zc);
             rays_to_images(zc);
@@ -1170,7 +1312,8 @@ struct Solver {
         RTV_REQUIRE(niter >= 0 && titer >= 0, "niter and titer must be >= 0");
         RTV_REQUIRE(std::isfinite(lam0) && std::isfinite(lam1) && lam1 != 0.f, "lambda1 must be finite and nonzero");
         for (int it = 0; it < niter; it++) {
-            if (use_graph && !prof) graph_iteration(titer, lam0, lam1);
+            if (ooc) one_iteration_ooc(titer, lam0, lam1);
+            else if (use_graph && !prof) graph_iteration(titer, lam0, lam1);
             else one_iteration(titer, lam0, lam1);
             iters_done++;
         }
@@ -1218,8 +1361,19 @@ struct Solver {
 
     void get_result(float *fres) {
         RTV_REQUIRE(fres, "null output pointer");
+        if (ooc) {
+            RTV_HIP(hipStreamSynchronize(st));
+            std::memcpy(fres, h_f[hf], (size_t)nz * M * n * n * sizeof(float));
+            return;
+        }
         RTV_HIP(hipMemcpyAsync(fres, d_f[ia].p, (size_t)nz * M * n * n * sizeof(float), hipMemcpyDeviceToHost, st));
         RTV_HIP(hipStreamSynchronize(st));
+    }
+    size_t dual_chunk() const { return (size_t)(nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1); }
+    void require_resident(const char *what) const {
+        if (ooc)
+            throw Error(-1, std::string(what) + " needs the state in device memory (this solver holds it in host "
+                                                "memory: out of core)");
     }
 
     // ------------------------------------------------------------ state (checkpoint / restore)
@@ -1228,6 +1382,7 @@ struct Solver {
     // iteration counter and history restart at zero.  Replaces the zero start of
     // rectv::run (rectv.cu:119-125) for callers that resume a saved state.
     void set_state(const float *f, const float *psi, const float *mu, float rho, float lam1) {
+        require_resident("set_state");
         RTV_REQUIRE(std::isfinite(rho) && rho > 0.f, "rho must be positive and finite");
         RTV_REQUIRE(std::isfinite(lam1) && lam1 != 0.f, "lambda1 must be finite and nonzero");
         const size_t V = (size_t)nz * M * n * n;
@@ -1247,6 +1402,7 @@ struct Solver {
     }
 
     void get_state(float *f, float *psi, float *mu, float *dchi, float *rho) {
+        require_resident("get_state");
         const size_t V = (size_t)nz * M * n * n;
         if (f) RTV_HIP(hipMemcpyAsync(f, d_f[ia].p, V * sizeof(float), hipMemcpyDeviceToHost, st));
         if (psi) RTV_HIP(hipMemcpyAsync(psi, d_psi.p, d_psi.n * sizeof(float4), hipMemcpyDeviceToHost, st));
@@ -1260,6 +1416,7 @@ struct Solver {
     // resident previous iterate, psi, mu, rho and data: fout = fcur - 1/2 div(rho grad
     // fcur + mu - rho psi) - 1/2 R^*(R fcur - g) (admm.cu:85-97).
     void apply_inner_step(const float *fcur, float *fout, float lam1) {
+        require_resident("apply_inner_step");
         RTV_REQUIRE(inputs, "inputs not set");
         RTV_REQUIRE(fcur && fout, "null array");
         const size_t V = (size_t)nz * M * n * n;
@@ -1276,6 +1433,7 @@ struct Solver {
     // (rectv.cu:224-232) and div(mu - rho psi) for the next iteration.  The resident
     // previous iterate is kept.  row receives the history row (HIST_W doubles).
     void apply_prox(const float *fcur, float lam0, float lam1, double *row) {
+        require_resident("apply_prox");
         RTV_REQUIRE(fcur, "null array");
         RTV_REQUIRE(std::isfinite(lam0) && std::isfinite(lam1) && lam1 != 0.f, "lambda1 must be finite and nonzero");
         const size_t V = (size_t)nz * M * n * n;
@@ -1361,6 +1519,7 @@ struct Solver {
     // psi = mu = 0, rho = 0.5 (the reference starts from f = 0, rectv.cu:119-125).
     void fbp_start() {
         RTV_REQUIRE(inputs, "inputs not set");
+        require_resident("the filtered back-projection start");
         reset();
         fbp_dev(d_g.p, d_f[ia].p);
         RTV_HIP(hipStreamSynchronize(st));
@@ -1593,6 +1752,17 @@ struct Solver {
         d_hist.release(t);
         d_iter.release(t);
         for (auto &b : d_f) b.release(t);
+        for (auto &h : h_f) {
+            if (h) (void)hipHostFree(h);
+            h = nullptr;
+        }
+        for (float4 **h : {&h_psi, &h_mu}) {
+            if (*h) (void)hipHostFree(*h);
+            *h = nullptr;
+        }
+        if (h_g) (void)hipHostFree(h_g);
+        h_g = nullptr;
+        d_acc.release(t);
         d_psi.release(t);
         d_mu.release(t);
         d_dchi.release(t);
