@@ -39,7 +39,19 @@ Pinning (what makes this oracle trustworthy, see tests/test_oracle_pins.py):
     plausible mistakes (time-reversed basis, inverted rho rule, dual-update sign,
     halo from the new iterate, rounded frame times) in copies of this module and
     checks that each fails one of these pins.
-Every function here has at least one of these checks.
+  * FBP filter, support mask, fbp (tests/test_oracle_fbp.py): a hand-counted
+    mask, the filter's symmetry and ramp limit, the filter equal to an explicit
+    circular convolution on the detector before adjR, an analytic disk
+    reconstructed with contrast 1, the warm start lowering the objective.
+  * pre-processing (tests/test_oracle_prep.py): dark/flat -log inverting a known
+    attenuation; the stripe removal's closed-form response to a single stripe;
+    the phase filter's unit gain at zero frequency and its large-alpha limit.
+    retrieve_phase restates tomopy's published retrieve_phase (tomopy.prep.phase,
+    called through tomocupy's command line of examples/tao/rec_line.txt:1; no
+    version is pinned by the reference and neither package is installed here):
+    its padding widths, reciprocal grid and max normalisation are parity
+    unpinned beyond the properties above.
+Every other function here has at least one of these checks.
 
 Precision: FP64 throughout by default (``dtype=np.float64``).  With
 ``dtype=np.float32`` the same code runs in FP32/complex64; that variant is the
