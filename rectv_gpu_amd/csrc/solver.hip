@@ -607,6 +607,12 @@ struct Solver {
     float4 *h_psi = nullptr, *h_mu = nullptr;
     float *h_g = nullptr;
     DBuf<double> d_acc;  // norms summed over the windows
+    // a second window slot, so that the copies of the neighbouring windows run on
+    // their own streams while a window computes (slot 0: d_f, d_psi, d_mu, d_dchi, d_g)
+    DBuf<float> d_f2[3], d_dchi2, d_g2;
+    DBuf<float4> d_psi2, d_mu2;
+    hipStream_t s_in = nullptr, s_out = nullptr;
+    hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_comp[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
     // penalty rho, its rule and the per-iteration norms live on the device
     // (k_rho_rule), so an outer iteration needs no host round trip
     DBuf<float> d_rho;
@@ -718,6 +724,8 @@ struct Solver {
         const size_t V = (size_t)nz * S, V4 = (size_t)nch * D;
         const size_t per_slice = (size_t)MP * n * N * 8 + (size_t)MP * N * N * 8 + (size_t)MP * n * n * 8 +
                                  (size_t)nt * n * 24 + (size_t)nt * n * 4;
+        // device bytes per chunk of state: f three times and div chi, psi, mu, data; out of
+        // core a window is held twice (two slots)
         const size_t per_chunk = 4 * (size_t)nzp * S * 4 + 2 * D * 16 + (size_t)nzp * nt * n * 4;
         size_t freeb = 0, totb = 0;
         RTV_HIP(hipMemGetInfo(&freeb, &totb));
@@ -726,7 +734,7 @@ struct Solver {
             if (std::atoi(e) > 0) W = std::min(nch, std::atoi(e));
         } else if ((double)per_chunk * nch + (double)per_slice * nzp > 0.9 * (double)freeb) {
             W = (int)std::max(1.0, std::min((double)nch, std::floor(0.55 * (double)freeb /
-                                                                     ((double)per_chunk + (double)per_slice * nzp))));
+                                                                     (2.0 * per_chunk + (double)per_slice * nzp))));
         }
         ooc = W < nch;
         const size_t ws = ooc ? (size_t)W * nzp + 2 : (size_t)nz;  // f slices on the device (window + 2 halos)
@@ -748,6 +756,16 @@ struct Solver {
             host((void **)&h_mu, V4 * sizeof(float4));
             host((void **)&h_g, (size_t)nz * nt * n * sizeof(float));
             d_acc.alloc(8, bytes);
+            for (auto &b : d_f2) b.alloc(ws * S, bytes);
+            d_psi2.alloc(W * D, bytes);
+            d_mu2.alloc(W * D, bytes);
+            d_dchi2.alloc((size_t)W * nzp * S, bytes);
+            d_g2.alloc((size_t)W * nzp * nt * n, bytes);
+            RTV_HIP(hipStreamCreateWithFlags(&s_in, hipStreamNonBlocking));
+            RTV_HIP(hipStreamCreateWithFlags(&s_out, hipStreamNonBlocking));
+            for (int q = 0; q < 2; q++)
+                for (hipEvent_t *e : {&ev_in[q], &ev_comp[q], &ev_out[q]})
+                    RTV_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
         }
         v_psi = d_psi.p;
         v_mu = d_mu.p;
@@ -1196,31 +1214,47 @@ struct Solver {
         const TvArgs ta = tv_args(lam1, lam0);
         const float *hin = h_f[hf];
         float *hout = h_f[hf ^ 1];
+        struct Slot {
+            float *f[3];
+            float4 *psi, *mu;
+            float *dchi, *g;
+        };
+        const Slot slot[2] = {{{d_f[0].p, d_f[1].p, d_f[2].p}, d_psi.p, d_mu.p, d_dchi.p, d_g.p},
+                              {{d_f2[0].p, d_f2[1].p, d_f2[2].p}, d_psi2.p, d_mu2.p, d_dchi2.p, d_g2.p}};
         RTV_HIP(hipMemsetAsync(d_acc.p, 0, 8 * sizeof(double), st));
-        for (int g0 = 0; g0 < nch; g0 += W) {
+        // window k uses slot k & 1: its uploads (s_in) wait for the downloads of window
+        // k - 2 from that slot, its kernels (st) for its uploads, its downloads (s_out)
+        // for its kernels; the three streams overlap neighbouring windows
+        int k = 0;
+        for (int g0 = 0; g0 < nch; g0 += W, k++) {
+            const int q = k & 1;
+            const Slot &sl = slot[q];
             const int w = std::min(W, nch - g0);
             const size_t z0 = (size_t)g0 * nzp, zw = (size_t)w * nzp;
             const size_t lo = g0 > 0 ? z0 - 1 : z0, hi = g0 + w < nch ? z0 + zw + 1 : z0 + zw;
-            RTV_HIP(hipMemcpyAsync(d_f[0].p + (lo + 1 - z0) * S, hin + lo * S, (hi - lo) * S * sizeof(float),
-                                   hipMemcpyHostToDevice, st));
-            RTV_HIP(hipMemcpyAsync(d_psi.p, h_psi + g0 * D, w * D * sizeof(float4), hipMemcpyHostToDevice, st));
-            RTV_HIP(hipMemcpyAsync(d_mu.p, h_mu + g0 * D, w * D * sizeof(float4), hipMemcpyHostToDevice, st));
-            RTV_HIP(hipMemcpyAsync(d_g.p, h_g + z0 * G1, zw * G1 * sizeof(float), hipMemcpyHostToDevice, st));
+            if (k >= 2) RTV_HIP(hipStreamWaitEvent(s_in, ev_out[q], 0));
+            RTV_HIP(hipMemcpyAsync(sl.f[0] + (lo + 1 - z0) * S, hin + lo * S, (hi - lo) * S * sizeof(float),
+                                   hipMemcpyHostToDevice, s_in));
+            RTV_HIP(hipMemcpyAsync(sl.psi, h_psi + g0 * D, w * D * sizeof(float4), hipMemcpyHostToDevice, s_in));
+            RTV_HIP(hipMemcpyAsync(sl.mu, h_mu + g0 * D, w * D * sizeof(float4), hipMemcpyHostToDevice, s_in));
+            RTV_HIP(hipMemcpyAsync(sl.g, h_g + z0 * G1, zw * G1 * sizeof(float), hipMemcpyHostToDevice, s_in));
+            RTV_HIP(hipEventRecord(ev_in[q], s_in));
+            RTV_HIP(hipStreamWaitEvent(st, ev_in[q], 0));
             // views addressed by global chunk / slice index
             float *fv[3];
-            for (int b = 0; b < 3; b++) fv[b] = shift_back(d_f[b].p + S, z0 * S);
-            v_psi = shift_back(d_psi.p, g0 * D);
-            v_mu = shift_back(d_mu.p, g0 * D);
-            v_dchi = shift_back(d_dchi.p, z0 * S);
-            v_g = shift_back(d_g.p, z0 * G1);
+            for (int b = 0; b < 3; b++) fv[b] = shift_back(sl.f[b] + S, z0 * S);
+            v_psi = shift_back(sl.psi, g0 * D);
+            v_mu = shift_back(sl.mu, g0 * D);
+            v_dchi = shift_back(sl.dchi, z0 * S);
+            v_g = shift_back(sl.g, z0 * G1);
             c_lo = g0;
             c_n = w;
             divchi(ta);
             int cur = 0;
-            for (int k = 0; k < titer; k++) {
-                const int nxt = (k == 0) ? 1 : (cur == 1 ? 2 : 1);
-                inner_step(fv[cur], fv[0], fv[nxt], ta, k == 0);
-                if (k == 0)
+            for (int it = 0; it < titer; it++) {
+                const int nxt = (it == 0) ? 1 : (cur == 1 ? 2 : 1);
+                inner_step(fv[cur], fv[0], fv[nxt], ta, it == 0);
+                if (it == 0)
                     launch("accumulate", [&] {
                         hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, st, d_acc.p + 5, d_red.p + 5, 1);
                     });
@@ -1230,9 +1264,12 @@ struct Solver {
             launch("accumulate", [&] {
                 hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, st, d_acc.p, d_red.p, PROX_W);
             });
-            RTV_HIP(hipMemcpyAsync(hout + z0 * S, d_f[cur].p + S, zw * S * sizeof(float), hipMemcpyDeviceToHost, st));
-            RTV_HIP(hipMemcpyAsync(h_psi + g0 * D, d_psi.p, w * D * sizeof(float4), hipMemcpyDeviceToHost, st));
-            RTV_HIP(hipMemcpyAsync(h_mu + g0 * D, d_mu.p, w * D * sizeof(float4), hipMemcpyDeviceToHost, st));
+            RTV_HIP(hipEventRecord(ev_comp[q], st));
+            RTV_HIP(hipStreamWaitEvent(s_out, ev_comp[q], 0));
+            RTV_HIP(hipMemcpyAsync(hout + z0 * S, sl.f[cur] + S, zw * S * sizeof(float), hipMemcpyDeviceToHost, s_out));
+            RTV_HIP(hipMemcpyAsync(h_psi + g0 * D, sl.psi, w * D * sizeof(float4), hipMemcpyDeviceToHost, s_out));
+            RTV_HIP(hipMemcpyAsync(h_mu + g0 * D, sl.mu, w * D * sizeof(float4), hipMemcpyDeviceToHost, s_out));
+            RTV_HIP(hipEventRecord(ev_out[q], s_out));
         }
         v_psi = d_psi.p;
         v_mu = d_mu.p;
@@ -1244,8 +1281,10 @@ struct Solver {
         rho_rule(titer > 0);
         hf ^= 1;
         // the host arrays are read and written by the copies above: finish them before
-        // the caller can read the result or the next iteration overwrites hout
+        // the caller can read the result or the next iteration reads hout as its input
+        RTV_HIP(hipStreamSynchronize(s_in));
         RTV_HIP(hipStreamSynchronize(st));
+        RTV_HIP(hipStreamSynchronize(s_out));
     }
 
     // One inner ADMM step (admm.cu:85-97) over all chunks: nxt = cur - 1/2 div(rho
@@ -1763,6 +1802,23 @@ struct Solver {
         if (h_g) (void)hipHostFree(h_g);
         h_g = nullptr;
         d_acc.release(t);
+        for (auto &b : d_f2) b.release(t);
+        d_psi2.release(t);
+        d_mu2.release(t);
+        d_dchi2.release(t);
+        d_g2.release(t);
+        for (hipStream_t *q : {&s_in, &s_out}) {
+            if (*q) {
+                (void)hipStreamSynchronize(*q);
+                (void)hipStreamDestroy(*q);
+            }
+            *q = nullptr;
+        }
+        for (int q = 0; q < 2; q++)
+            for (hipEvent_t *e : {&ev_in[q], &ev_comp[q], &ev_out[q]}) {
+                if (*e) (void)hipEventDestroy(*e);
+                *e = nullptr;
+            }
         d_psi.release(t);
         d_mu.release(t);
         d_dchi.release(t);
