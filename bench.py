@@ -300,9 +300,9 @@ def main():
     ap.add_argument("--job-niter", type=int, default=64, help="outer iterations per batched / served job")
     ap.add_argument("--job-titer", type=int, default=4)
     ap.add_argument("--rate", type=float, default=20.0, help="serving arrival rate (jobs/s)")
-    ap.add_argument("--light-rate", type=float, default=2.0,
+    ap.add_argument("--light-rate", type=float, default=0.5,
                     help="arrival rate of the second, below-capacity serving run (jobs/s; 0 skips it)")
-    ap.add_argument("--light-jobs", type=int, default=24)
+    ap.add_argument("--light-jobs", type=int, default=12)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: the solver drives exactly one GPU; --gpus %d is not supported "

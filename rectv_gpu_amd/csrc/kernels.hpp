@@ -47,7 +47,8 @@ __host__ __device__ constexpr int tiles_per_side(int N) { return (N + TS - 1) / 
 __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 
 // H holds each (slice, pair) spectrum as groups of g adjacent columns kx, each
-// group [n rows y][g]; g = 2n is the plain row-major layout.  The forward
+// group [n rows y][g] (ceil(2n/g) groups: for odd n the last group is half used);
+// g = 2n is the plain row-major layout.  The forward
 // direction uses g = HG: the column pass then reads contiguous blocks, and the row
 // pass writes g-element row segments, which the TT rows of one workgroup complete
 // to whole cache lines.  The adjoint uses g = HGA (0: row-major): its column pass
@@ -62,7 +63,7 @@ __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 constexpr int HG = RTV_HG, HGA = RTV_HGA;
 __host__ __device__ constexpr int h_group_adj(int N) { return HGA > 0 ? HGA : N; }
 __device__ __forceinline__ size_t h_index(size_t zj, int n, int y, int kx, int g) {
-    return ((zj * (size_t)(2 * n / g) + (size_t)(kx / g)) * (size_t)n + y) * g + (kx % g);
+    return ((zj * (size_t)((2 * n + g - 1) / g) + (size_t)(kx / g)) * (size_t)n + y) * g + (kx % g);
 }
 // element kx = jl + m * TPT of row y (row passes of a power-of-two length)
 template <int TPT, int G>

@@ -665,7 +665,7 @@ struct Solver {
     Solver(int n_, int nt_, int m_, int nz_, int nzp_, int ngpus_) : n(n_), nt(nt_), M(m_), nz(nz_), nzp(nzp_) {
         RTV_REQUIRE(n > 0 && nt > 0 && M > 0 && nz > 0 && nzp > 0, "all sizes must be positive");
         RTV_REQUIRE(nz % nzp == 0, "nz must be a multiple of nzp (the reference silently drops the remainder)");
-        RTV_REQUIRE(n % 2 == 0 && n >= 16, "n must be even and at least 16");
+        RTV_REQUIRE(n >= 16, "n must be at least 16");
         nch = nz / nzp;
         MP = (M + 1) / 2;
         ngpus = std::max(1, std::min(ngpus_, nch));
@@ -684,7 +684,7 @@ struct Solver {
         const size_t V = (size_t)nz * M * n * n;
         const size_t V4 = nch * (nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1);
         const size_t state = 3 * V * 4 + 2 * V4 * 16 + V * 4 + (size_t)nz * nt * n * 4;
-        const size_t per_slice = MP * n * N * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 24 + (size_t)nt * n * 4;
+        const size_t per_slice = MP * n * (N + 2) * 8 + MP * N * N * 8 + MP * n * n * 8 + (size_t)nt * n * 24 + (size_t)nt * n * 4;
         const size_t ni = 2 * (size_t)nt * n, ntile = (size_t)tiles_per_side(N) * tiles_per_side(N);
         // geo 16 B, gather slots <= 16 ints, adjoint lists <= 4 records of 16 + 8 B and
         // block lists <= 9 bytes per item (16 B spare); per adjoint chunk its range, 16
@@ -722,7 +722,7 @@ struct Solver {
         // chunk of operator scratch would not fit in free device memory
         const size_t S = (size_t)M * n * n, D = (size_t)(nzp + 1) * (M + 1) * (size_t)(n + 1) * (n + 1);
         const size_t V = (size_t)nz * S, V4 = (size_t)nch * D;
-        const size_t per_slice = (size_t)MP * n * N * 8 + (size_t)MP * N * N * 8 + (size_t)MP * n * n * 8 +
+        const size_t per_slice = (size_t)MP * n * (N + 2) * 8 + (size_t)MP * N * N * 8 + (size_t)MP * n * n * 8 +
                                  (size_t)nt * n * 24 + (size_t)nt * n * 4;
         // device bytes per chunk of state: f three times and div chi, psi, mu, data; out of
         // core a window is held twice (two slots)
@@ -779,7 +779,7 @@ struct Solver {
         if (const char *e = std::getenv("RECTV_SCRATCH_MB")) budget = std::atof(e) * 1048576.0;
         cb = (int)std::max(1.0, std::min((double)W, std::floor(budget / ((double)per_slice * nzp))));
         zb = cb * nzp;
-        d_H.alloc((size_t)zb * MP * n * N, bytes);
+        d_H.alloc((size_t)zb * MP * n * ((N + 3) & ~3), bytes);  // whole 4-column groups (odd n: 2n = 2 mod 4)
         d_G.alloc((size_t)zb * MP * N * N, bytes);
         d_v.alloc((size_t)zb * MP * n * n, bytes);
         d_gq.alloc((size_t)zb * nt * n * 2, bytes);

@@ -38,8 +38,8 @@ def test_failure_paths_report_status_and_message():
     # invalid shape is rejected before any device work (nz % nzp != 0)
     st = L.rtv_create(64, 32, 4, 5, 2, 1, ctypes.byref(h))
     assert st == -1 and b"multiple of nzp" in L.rtv_last_error()
-    st = L.rtv_create(61, 32, 4, 4, 2, 1, ctypes.byref(h))
-    assert st == -1 and b"even" in L.rtv_last_error()
+    st = L.rtv_create(8, 32, 4, 4, 2, 1, ctypes.byref(h))
+    assert st == -1 and b"at least 16" in L.rtv_last_error()
     # a null / freed handle is a state error, not a crash
     assert L.rtv_reset(None) == -4
     assert L.rtv_free(None) == 0

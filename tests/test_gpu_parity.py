@@ -105,6 +105,18 @@ def _case(name):
         n, nt, m, nz = 136, 12, 2, 1
         th = B.angles(6, 2)
         phi, center = B.takephi(nt, m), n / 2
+    elif name == "n45_odd":               # odd n: 2n = 90 = 2 mod 4 (a half-used last column group of H)
+        n, nt, m, nz = 45, 20, 4, 2
+        th = B.angles(10, 2)
+        phi, center = B.takephi(nt, m), n / 2
+    elif name == "n17_odd_prime":         # n = 17 (radix 17), 2n = 34: narrow tiles 2 cells wide
+        n, nt, m, nz = 17, 12, 3, 1
+        th = np.sort(rng.uniform(0, 2 * np.pi, nt)).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
+    elif name == "n63_odd":               # n = 63 = 9*7, 2n = 126
+        n, nt, m, nz = 63, 18, 5, 1
+        th = B.angles(9, 2)
+        phi, center = B.takephi(nt, m), n / 2 - 1.0
     elif name == "n56_radix7_small":      # n = 56 = 8*7, 2n = 112 = 16*7
         n, nt, m, nz = 56, 15, 3, 2
         th = np.sort(rng.uniform(0, 3 * np.pi, nt)).astype(np.float32)
@@ -117,7 +129,8 @@ def _case(name):
 CASES = ["n64_rand", "n72_mixed_radix_m3", "n384_mixed_radix", "cfg_small", "n64_m20_two_blocks",
          "n1024_column_rounds", "n2048_column_rounds", "n480_radix5", "n504_radix7", "n40_radix5_small",
          "n56_radix7_small", "n64_m40_pair_groups", "n32_m65_pair_groups", "n36_narrow_tile",
-         "n50_three_tiles", "n100_narrow_tile", "n44_radix11", "n34_radix17", "n136_radix17"]
+         "n50_three_tiles", "n100_narrow_tile", "n44_radix11", "n34_radix17", "n136_radix17", "n45_odd",
+         "n17_odd_prime", "n63_odd"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -278,6 +291,7 @@ def _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed, basis="fourier", lam
     (32, 4, 2, 2, 16, 19, 3, 2),     # m = 19: two MFMA column blocks, odd m
     (32, 4, 2, 2, 24, 41, 3, 2),     # m = 41: frame groups 32 + 9, pair groups 16 + 5
     (50, 4, 2, 2, 16, 4, 3, 2),      # n = 50: 2n = 100, narrow last tiles
+    (45, 4, 2, 2, 16, 4, 3, 2),      # odd n
 ])
 def test_run_parity_small(gpu, n, nz, nzp, nrot, ntr, m, niter, titer):
     """Result, rho sequence, r, s, ||f(k+1)-f(k)||^2, ||f(k+1)||^2 and both objective

@@ -54,7 +54,7 @@ def _state(seed, n, nt, m, nz, nzp):
 # m odd (a half-empty last frame pair); three chunks (halo slabs both sides);
 # m = 67: frame groups of 32 + 32 + 3 in the TV kernels, pair groups of 16 + 16 + 2
 CASES = [(32, 24, 5, 6, 2, 0.75, 2.5), (40, 20, 4, 4, 2, 2.0, 4.0), (64, 16, 3, 3, 1, 0.25, 1.0),
-         (32, 72, 67, 4, 2, 1.5, 3.0), (36, 20, 4, 4, 2, 0.5, 2.0)]
+         (32, 72, 67, 4, 2, 1.5, 3.0), (36, 20, 4, 4, 2, 0.5, 2.0), (45, 16, 3, 4, 2, 1.0, 2.0)]
 
 
 @pytest.mark.parametrize("n,nt,m,nz,nzp,rho,lam1", CASES)
