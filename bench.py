@@ -374,6 +374,19 @@ def main():
             "traffic": measured_traffic(a.config, dom), "alg_bytes_per_launch": db}
     B = byte_model(n, nt, m, nz, nzp, a.titer)
     Bp = byte_model(n, nt, m, nz, nzp, a.titer, pairs=True)
+    # the API default titer = 1 (solver.py:47) on the same solver and data, as a secondary figure
+    t1_out = None
+    if a.titer != 1 and a.steps > 0:
+        s.reset()
+        s.iterate(1, 1, lam0, lam1)
+        s.synchronize()
+        u0 = time.perf_counter()
+        s.iterate(a.steps, 1, lam0, lam1)
+        s.synchronize()
+        u1 = time.perf_counter()
+        t1_out = {"titer": 1, "ms_per_step": round((u1 - u0) * 1e3 / a.steps, 2),
+                  "value": round(vox * a.steps / (u1 - u0), 1),
+                  "iteration_hbm_frac": round(byte_model(n, nt, m, nz, nzp, 1) / ((u1 - u0) / a.steps) / HBM_PEAK, 4)}
     s.free()
     copy_bw = copy_probe()
     out = {
@@ -395,6 +408,7 @@ def main():
         "iteration_copy_frac_pairs": round(Bp / (ms * 1e-3) / copy_bw, 4) if copy_bw > 0 else None,
         "roofline": roof,
         "kernels": kern,
+        "titer1": t1_out,
     }
     if a.mode == "all":
         out["batched"] = batched_bench(a.jobs, a.job_niter, a.job_titer)
