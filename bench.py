@@ -40,7 +40,7 @@ def kernel_bytes(name, n, nt, m, zc, nzp):
         "adj_cols": S * (N * N + n * N) * 8,
         "adj_rows": S * (n * N + n * n) * 8,
         "update": V * (4 + 4 + 4 + 4),         # f in, div(chi) in, adjoint frames in, f out
-        "divchi": 32 * V4 + 4 * V,             # psi, mu in; div(mu - rho psi) out
+        "divchi": 16 * V4 + 4 * V,             # chi = mu - rho psi in (and mu when the rule changed rho); div chi out
         "prox": 8 * V + 48 * V4,
     }.get(name)
 
@@ -272,7 +272,7 @@ def served_bench(count, niter, titer, rate, seed=7, max_active=16, sample_frac=0
         plain.niter = jobs[i].iterations  # the served job may have stopped early
         r = batch.solve_sequential([plain])[0].result
         diff = max(diff, float(np.linalg.norm(jobs[i].result.astype(np.float64) - r) / max(np.linalg.norm(r), 1e-30)))
-    return {"workload": "%d jobs (batched mix + 5%% N=1024 Nz=32 M=16), Poisson arrivals %.0f jobs/s, niter=%d "
+    return {"workload": "%d jobs (batched mix + 5%% N=1024 Nz=32 M=16), Poisson arrivals %g jobs/s, niter=%d "
                         "or relative change < 1e-4, titer=%d, max %d active" % (count, rate, niter, titer, max_active),
             "value": round(vox / (t1 - t0), 1), "unit": "voxel-iter/s", "wall_s": round(t1 - t0, 3),
             "latency": lat, "jobs": count, "errors": len(errors), "peak_active": srv.peak_active,

@@ -1115,7 +1115,11 @@ __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i
 // the rho rule.  Thread per (chunk, y, x) column; the t and z neighbours are the
 // previous frame's / slice's own values and are carried.
 // Frames i0 .. i0 + MM - 1 (one launch per group of MM frames when M > MM; i0 even).
-template <int MM>
+// With CHI the first array holds chi0 = mu - rho0 psi as k_prox<true> stored it,
+// rho0 = a.rho[1] the penalty of that prox and rho = a.rho[0] the one after the
+// rule: chi = mu - rho psi = (1 - c) mu + c chi0 with c = rho / rho0 (1, 2 or 1/2),
+// so mu is read only when the rule changed rho.
+template <int MM, bool CHI>
 __global__ void __launch_bounds__(TV_THREADS)
 k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *__restrict__ dchi, TvArgs a, int i0) {
     const TvTile tt = tv_tile(a.n, a.n, a.chunk0);
@@ -1131,9 +1135,26 @@ k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *_
     mu += (size_t)i0 * (n + 1) * (n + 1);
     dchi += (size_t)i0 * plane;
     const int Mg = M - i0;
+    const float cr = CHI ? r / a.rho[1] : 1.f;
+    const bool mix = CHI && cr != 1.f;  // uniform
+    const float cm = 1.f - cr;
     auto chi = [&](size_t d) {
-        const float4 p = psi[d], m = mu[d];
+        const float4 p = psi[d];
+        if (CHI) {
+            if (!mix) return p;
+            const float4 m = mu[d];
+            return make_float4(cm * m.x + cr * p.x, cm * m.y + cr * p.y, cm * m.z + cr * p.z, cm * m.w + cr * p.w);
+        }
+        const float4 m = mu[d];
         return make_float4(m.x - r * p.x, m.y - r * p.y, m.z - r * p.z, m.w - r * p.w);
+    };
+    auto chi1 = [&](size_t d, int k) {  // component k of chi at entry d
+        const float *pp = reinterpret_cast<const float *>(psi + d) + k;
+        if (CHI) {
+            if (!mix) return *pp;
+            return cm * reinterpret_cast<const float *>(mu + d)[k] + cr * *pp;
+        }
+        return reinterpret_cast<const float *>(mu + d)[k] - r * *pp;
     };
     float czc[MM];  // chi_z of the previous slice plane (ghost plane z' = 0 first)
 #pragma unroll
@@ -1147,8 +1168,8 @@ k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *_
             if (i < Mg) {
                 const size_t d = dual_idx(a, c, zl + 1, i + 1, y + 1, x + 1);
                 const float4 c0 = chi(d);
-                const float cxm = mu[d - 1].x - r * psi[d - 1].x;
-                const float cym = mu[d - (n + 1)].y - r * psi[d - (n + 1)].y;
+                const float cxm = chi1(d - 1, 0);
+                const float cym = chi1(d - (n + 1), 1);
                 o[(size_t)i * plane] = a.ks * (cxm - c0.x) + a.ks * (cym - c0.y) + a.kt * (ctc - c0.z) +
                                        a.ks * (czc[i] - c0.w);
                 ctc = c0.z;
@@ -1261,12 +1282,17 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
 // mostly cache hits: blocks run plane after plane over one band of rows, so the +t
 // and +z planes are read again by the next blocks).  Halo slabs come from A
 // (rectv.cu:170-171).
+// psi itself is needed only through chi = mu' - rho psi (the dual part of the next
+// inner steps, k_divchi): with CHI the kernel stores chi in the psi array, so that
+// k_divchi reads one dual array instead of two; psi = (mu' - chi) / rho is
+// recovered where a caller asks for it (state export).
 constexpr int PROX_W = 5;  // partial sums per workgroup: r, s, ||fm - f_old||^2, ||fm||^2, sum |h2|
 constexpr int PX_ROWS = 16;  // rows y' per workgroup
 #ifndef RTV_PX_WAVES
 #define RTV_PX_WAVES 16
 #endif
 constexpr int PX_WAVES = RTV_PX_WAVES;  // waves per workgroup
+template <bool CHI>
 __global__ void __launch_bounds__(64 * PX_WAVES)
 k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__restrict__ psi,
             float4 *__restrict__ mu, TvArgs a, double *__restrict__ partial) {
@@ -1350,7 +1376,11 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
             m4.y += r * hy - r * p4.y;
             m4.z += r * ht - r * p4.z;
             m4.w += r * hz - r * p4.w;
-            __builtin_nontemporal_store((v4f){p4.x, p4.y, p4.z, p4.w}, (v4f *)(psi + d));
+            if (CHI)  // chi = mu' - rho psi in place of psi: k_divchi then reads one dual array
+                __builtin_nontemporal_store((v4f){m4.x - r * p4.x, m4.y - r * p4.y, m4.z - r * p4.z, m4.w - r * p4.w},
+                                            (v4f *)(psi + d));
+            else
+                __builtin_nontemporal_store((v4f){p4.x, p4.y, p4.z, p4.w}, (v4f *)(psi + d));
             __builtin_nontemporal_store((v4f){m4.x, m4.y, m4.z, m4.w}, (v4f *)(mu + d));
             const float ex = hx - p4.x, ey = hy - p4.y, et = ht - p4.z, ez = hz - p4.w;
             sr += (double)ex * ex + (double)ey * ey + (double)et * et + (double)ez * ez;
@@ -1393,8 +1423,10 @@ __global__ void k_accumulate(double *__restrict__ dst, const double *__restrict_
     const int i = threadIdx.x;
     if (i < cnt) dst[i] += src[i];
 }
+// rho_chi (may be null): receives the penalty used by the prox that formed the
+// stored chi (k_divchi<.., true> rescales chi when the rule changed rho).
 __global__ void k_rho_rule(const double *__restrict__ red, float *__restrict__ rho, double *__restrict__ hist,
-                           int *__restrict__ iter, int cap, int has_data) {
+                           int *__restrict__ iter, int cap, int has_data, float *__restrict__ rho_chi) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const float rh = *rho;
     const double r = red[0], s = (double)rh * rh * red[1];
@@ -1411,6 +1443,7 @@ __global__ void k_rho_rule(const double *__restrict__ red, float *__restrict__ r
     if (r > 10 * s) nr = rh * 2;
     else if (s > 10 * r) nr = rh * 0.5f;
     *rho = nr;
+    if (rho_chi) *rho_chi = rh;
     *iter = it + 1;
 }
 
@@ -1452,6 +1485,14 @@ __global__ void k_grad(const float *__restrict__ fext, float4 *__restrict__ h2, 
     const float e0 = E(zq, iq, yq, xq);
     h2[p] = make_float4(gs * (E(zq, iq, yq, xq + 1) - e0), gs * (E(zq, iq, yq + 1, xq) - e0),
                         gt * (E(zq, iq + 1, yq, xq) - e0), gs * (E(zq + 1, iq, yq, xq) - e0));
+}
+
+// State import: chi = mu - rho psi in place of psi (the dual array of k_prox<true>).
+__global__ void k_form_chi(float4 *__restrict__ psi, const float4 *__restrict__ mu, float rho, size_t cnt) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const float4 p = psi[i], m = mu[i];
+    psi[i] = make_float4(m.x - rho * p.x, m.y - rho * p.y, m.z - rho * p.z, m.w - rho * p.w);
 }
 
 // Standalone divergence (div, kernels_operators.cuh:46-73): f += tau * div(h2).

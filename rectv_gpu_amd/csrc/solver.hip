@@ -584,6 +584,8 @@ struct Solver {
     // state
     DBuf<float> d_f[3];
     int ia = 0;
+    // d_psi holds chi = mu - rho0 psi (rho0 = d_rho[1], the penalty of the prox that
+    // formed it), not psi: the dual part of the inner steps needs psi only through chi
     DBuf<float4> d_psi, d_mu;
     DBuf<float> d_dchi;  // div(mu - rho psi) for the current outer iteration
     DBuf<float> d_g;
@@ -790,7 +792,7 @@ struct Solver {
         d_part.alloc(PROX_W * std::max(pb, rb) + 16, bytes);
         d_rpart.alloc((size_t)((nch + cb - 1) / cb) * rb + 1, bytes);
         d_red.alloc(8, bytes);
-        d_rho.alloc(1, bytes);
+        d_rho.alloc(2, bytes);  // rho, and the rho the stored chi was formed with
         d_hist.alloc((size_t)HIST_CAP * HIST_W, bytes);
         d_iter.alloc(1, bytes);
         RTV_HIP(hipStreamSynchronize(st));
@@ -962,9 +964,9 @@ struct Solver {
         RTV_HIP(hipMemsetAsync(d_psi.p, 0, d_psi.n * sizeof(float4), st));
         RTV_HIP(hipMemsetAsync(d_mu.p, 0, d_mu.n * sizeof(float4), st));
         RTV_HIP(hipMemsetAsync(d_dchi.p, 0, d_dchi.n * sizeof(float), st));  // psi = mu = 0
-        static const float rho0 = 0.5f;  // rectv.cu:125
+        static const float rho0[2] = {0.5f, 0.5f};  // rectv.cu:125
         static const int zero = 0;
-        RTV_HIP(hipMemcpyAsync(d_rho.p, &rho0, sizeof(float), hipMemcpyHostToDevice, st));
+        RTV_HIP(hipMemcpyAsync(d_rho.p, rho0, 2 * sizeof(float), hipMemcpyHostToDevice, st));
         RTV_HIP(hipMemcpyAsync(d_iter.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
         ia = 0;
         iters_done = 0;
@@ -1134,7 +1136,7 @@ struct Solver {
         TvArgs tb = ta;
         tb.chunk0 = c_lo;
         launch("prox", [&] {
-            hipLaunchKernelGGL(k_prox, grid, dim3(64 * PX_WAVES), 0, st, cur, A, v_psi, v_mu, tb, d_part.p);
+            hipLaunchKernelGGL(k_prox<true>, grid, dim3(64 * PX_WAVES), 0, st, cur, A, v_psi, v_mu, tb, d_part.p);
         });
         const int cnt = grid.x;
         launch("reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, d_part.p, d_red.p, cnt, PROX_W); });
@@ -1142,7 +1144,7 @@ struct Solver {
     void rho_rule(bool has_data) {
         launch("rho", [&] {
             hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, st, d_red.p, d_rho.p, d_hist.p, d_iter.p, HIST_CAP,
-                               has_data ? 1 : 0);
+                               has_data ? 1 : 0, d_rho.p + 1);
         });
     }
     void prox(const float *cur, const float *A, const TvArgs &ta, bool has_data) {
@@ -1156,7 +1158,7 @@ struct Solver {
         tb.chunk0 = c_lo;
         for (int i0 = 0; i0 < M; i0 += MM)
             launch("divchi", [&] {
-                hipLaunchKernelGGL(k_divchi<MM>, grid, dim3(TV_THREADS), 0, st, v_psi, v_mu, v_dchi, tb, i0);
+                hipLaunchKernelGGL((k_divchi<MM, true>), grid, dim3(TV_THREADS), 0, st, v_psi, v_mu, v_dchi, tb, i0);
             });
     }
     void divchi(const TvArgs &ta) {
@@ -1420,6 +1422,8 @@ struct Solver {
     // (NULL: zero); rho > 0.  div(mu - rho psi) is formed for lambda1, the outer
     // iteration counter and history restart at zero.  Replaces the zero start of
     // rectv::run (rectv.cu:119-125) for callers that resume a saved state.
+    // The solver holds chi = mu - rho psi in place of psi (k_prox<true>): the import
+    // forms it, the export recovers psi = (mu - chi) / rho (to FP32 rounding).
     void set_state(const float *f, const float *psi, const float *mu, float rho, float lam1) {
         require_resident("set_state");
         RTV_REQUIRE(std::isfinite(rho) && rho > 0.f, "rho must be positive and finite");
@@ -1431,8 +1435,12 @@ struct Solver {
         else RTV_HIP(hipMemsetAsync(d_psi.p, 0, d_psi.n * sizeof(float4), st));
         if (mu) RTV_HIP(hipMemcpyAsync(d_mu.p, mu, d_mu.n * sizeof(float4), hipMemcpyHostToDevice, st));
         else RTV_HIP(hipMemsetAsync(d_mu.p, 0, d_mu.n * sizeof(float4), st));
+        hipLaunchKernelGGL(k_form_chi, dim3((unsigned)((d_psi.n + 255) / 256)), dim3(256), 0, st, d_psi.p, d_mu.p, rho,
+                           d_psi.n);
+        RTV_HIP(hipGetLastError());
         static const int zero = 0;
-        RTV_HIP(hipMemcpyAsync(d_rho.p, &rho, sizeof(float), hipMemcpyHostToDevice, st));
+        const float rr[2] = {rho, rho};
+        RTV_HIP(hipMemcpyAsync(d_rho.p, rr, 2 * sizeof(float), hipMemcpyHostToDevice, st));
         RTV_HIP(hipMemcpyAsync(d_iter.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
         iters_done = 0;
         divchi(tv_args(lam1, 0.f));
@@ -1444,11 +1452,27 @@ struct Solver {
         require_resident("get_state");
         const size_t V = (size_t)nz * M * n * n;
         if (f) RTV_HIP(hipMemcpyAsync(f, d_f[ia].p, V * sizeof(float), hipMemcpyDeviceToHost, st));
-        if (psi) RTV_HIP(hipMemcpyAsync(psi, d_psi.p, d_psi.n * sizeof(float4), hipMemcpyDeviceToHost, st));
         if (mu) RTV_HIP(hipMemcpyAsync(mu, d_mu.p, d_mu.n * sizeof(float4), hipMemcpyDeviceToHost, st));
         if (dchi) RTV_HIP(hipMemcpyAsync(dchi, d_dchi.p, V * sizeof(float), hipMemcpyDeviceToHost, st));
-        if (rho) RTV_HIP(hipMemcpyAsync(rho, d_rho.p, sizeof(float), hipMemcpyDeviceToHost, st));
+        float rr[2] = {0.f, 0.f};
+        RTV_HIP(hipMemcpyAsync(rr, d_rho.p, 2 * sizeof(float), hipMemcpyDeviceToHost, st));
         RTV_HIP(hipStreamSynchronize(st));
+        if (rho) *rho = rr[0];
+        if (psi) {
+            // psi = (mu - chi) / rho_chi, one chunk of dual entries at a time
+            const size_t D = dual_chunk();
+            std::vector<float4> hc(D), hm(D);
+            for (int c = 0; c < nch; c++) {
+                RTV_HIP(hipMemcpyAsync(hc.data(), d_psi.p + c * D, D * sizeof(float4), hipMemcpyDeviceToHost, st));
+                RTV_HIP(hipMemcpyAsync(hm.data(), d_mu.p + c * D, D * sizeof(float4), hipMemcpyDeviceToHost, st));
+                RTV_HIP(hipStreamSynchronize(st));
+                float4 *o = reinterpret_cast<float4 *>(psi) + c * D;
+                const float r0 = rr[1];
+                for (size_t i = 0; i < D; i++)
+                    o[i] = make_float4((hm[i].x - hc[i].x) / r0, (hm[i].y - hc[i].y) / r0, (hm[i].z - hc[i].z) / r0,
+                                       (hm[i].w - hc[i].w) / r0);
+            }
+        }
     }
 
     // One inner step of the hot path on a given iterate fcur (host), against the
@@ -1921,20 +1945,21 @@ static void prox_noalloc(const float *fcur, const float *fprev, float *psi, floa
     a.rho = rho;
     a.lam0 = lam0;
     float4 *p4 = reinterpret_cast<float4 *>(psi), *m4 = reinterpret_cast<float4 *>(mu);
-    hipLaunchKernelGGL(k_prox, dim3((unsigned)blocks), dim3(64 * PX_WAVES), 0, stream, fcur, fprev, p4, m4, a, part);
+    hipLaunchKernelGGL(k_prox<false>, dim3((unsigned)blocks), dim3(64 * PX_WAVES), 0, stream, fcur, fprev, p4, m4, a,
+                       part);
     RTV_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, stream, part, red, (int)blocks, PROX_W);
     RTV_HIP(hipGetLastError());
     // one history row (capacity 1: the counter's value does not matter)
-    hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, stream, red, rho, row, counter, 1, 0);
+    hipLaunchKernelGGL(k_rho_rule, dim3(1), dim3(64), 0, stream, red, rho, row, counter, 1, 0, nullptr);
     RTV_HIP(hipGetLastError());
     const dim3 grid(Solver::tv_blocks(n, nch)), thr(TV_THREADS);
-    if (m <= 4) hipLaunchKernelGGL(k_divchi<4>, grid, thr, 0, stream, p4, m4, dchi, a, 0);
-    else if (m <= 8) hipLaunchKernelGGL(k_divchi<8>, grid, thr, 0, stream, p4, m4, dchi, a, 0);
-    else if (m <= 16) hipLaunchKernelGGL(k_divchi<16>, grid, thr, 0, stream, p4, m4, dchi, a, 0);
+    if (m <= 4) hipLaunchKernelGGL((k_divchi<4, false>), grid, thr, 0, stream, p4, m4, dchi, a, 0);
+    else if (m <= 8) hipLaunchKernelGGL((k_divchi<8, false>), grid, thr, 0, stream, p4, m4, dchi, a, 0);
+    else if (m <= 16) hipLaunchKernelGGL((k_divchi<16, false>), grid, thr, 0, stream, p4, m4, dchi, a, 0);
     else
         for (int i0 = 0; i0 < m; i0 += 32)  // groups of 32 frames
-            hipLaunchKernelGGL(k_divchi<32>, grid, thr, 0, stream, p4, m4, dchi, a, i0);
+            hipLaunchKernelGGL((k_divchi<32, false>), grid, thr, 0, stream, p4, m4, dchi, a, i0);
     RTV_HIP(hipGetLastError());
 }
 

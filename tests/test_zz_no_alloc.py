@@ -2,7 +2,9 @@
 scratch in a fixed per-device array) against rtv_apply_prox (the same phase on a
 solver handle, with scratch the library allocates), through the C ABI: two calls
 in a row, the second with smaller sizes, outputs equal to the handle-based entry
-point's (the same kernels in the same order: bitwise); sizes beyond the scratch
+point's (the same phase: mu, rho and the history row bitwise; psi and div chi to
+FP32 rounding, since the handle's solver stores chi = mu - rho psi in place of
+psi and exports psi = (mu - chi) / rho); sizes beyond the scratch
 are rejected before any launch.  Device buffers of the caller come from hipMalloc
 through ctypes."""
 import ctypes
@@ -86,11 +88,16 @@ def test_noalloc_matches_handle_entry_point_twice(gpu):
         lam0 = 0.3 * rho
         row_h, st_h = _with_handle(n, m, nz, nzp, A, fcur, psi, mu, rho, lam0, lam1)
         out = _noalloc(n, m, nz, nzp, A, fcur, psi, mu, rho, lam0, lam1)
-        for key in ("psi", "mu", "dchi"):
-            np.testing.assert_array_equal(out[key], st_h[key], err_msg=key)
+        np.testing.assert_array_equal(out["mu"], st_h["mu"], err_msg="mu")
+        # the handle's solver keeps chi = mu - rho psi in place of psi: its psi comes back
+        # as (mu - chi) / rho and its div chi from chi, equal to FP32 rounding
+        for key in ("psi", "dchi"):
+            e = float(np.linalg.norm(out[key].astype(np.float64) - st_h[key]) / np.linalg.norm(st_h[key]))
+            say("  %s rel L2 %.2e" % (key, e))
+            assert e < 1e-6, key
         np.testing.assert_array_equal(out["rho"][0], np.float32(st_h["rho"]))
         np.testing.assert_array_equal(out["row"], row_h)  # NaN data term in both
-        say("  equal: psi, mu, dchi, rho, row", row_h)
+        say("  equal: mu, rho, row", row_h)
 
 
 def test_noalloc_rejects_sizes_beyond_its_scratch(gpu):
