@@ -45,6 +45,19 @@ def kernel_bytes(name, n, nt, m, zc, nzp):
     }.get(name)
 
 
+def kernel_flops(name, n, nt, m, zc):
+    """Algorithmic FLOPs of one launch of the window kernels over zc slices: every
+    polar sample has a direct and a mirrored item, each a 9 x 9-tap window sum over
+    the 2*ceil(M/2) real columns of the pair spectra (2 FLOPs per multiply-add).
+    The FFT and TV kernels are counted by bytes only."""
+    if name not in ("gather", "scatter"):
+        return None
+    return 2 * nt * n * zc * 81 * 2 * ((m + 1) // 2) * 2
+
+
+FP32_MFMA_PEAK = 157.3e12  # dense FP32 matrix FLOP/s of MI355X
+
+
 def byte_model(n, nt, m, nz, nzp, titer, pairs=False):
     """Compulsory HBM bytes of one outer iteration (fused-pipeline model).  With
     pairs=False the model counts the reference algorithm's M*Nz spectra; with
@@ -190,11 +203,12 @@ def oracle_sample_errors(jobs, mix):
     return out
 
 
-def batched_bench(count, niter, titer, workers=8):
+def batched_bench(count, niter, titer, workers=8, part="both"):
     """Batched-instances mix: `count` uneven jobs solved one after another with the
     plain API (Solver.recon each) and then together (batch.solve_batch); same
     inputs, same arithmetic.  Value = sum of n^2*Nz*M*niter over jobs / wall time,
-    host set-up (geometry tables, uploads) included in both."""
+    host set-up (geometry tables, uploads) included in both.  part = "sequential" or
+    "batched" runs only that half (for a kernel trace of one of them)."""
     from rectv_gpu_amd import synthetic, batch
     specs = synthetic.batch_specs(count)
     jobs0 = [synthetic.make_job(sp, niter, titer) for sp in specs]
@@ -202,6 +216,16 @@ def batched_bench(count, niter, titer, workers=8):
     vox = sum(j.voxels * j.niter for j in jobs_a)
     print("bench: batched mix of %d jobs generated (%.2f G voxel-iterations)" % (count, vox / 1e9),
           file=sys.stderr, flush=True)
+    if part != "both":
+        t0 = time.perf_counter()
+        if part == "sequential":
+            batch.solve_sequential(jobs_a)
+        else:
+            batch.solve_batch(jobs_a, workers=workers)
+        t = time.perf_counter() - t0
+        print("bench: %s %.2f s" % (part, t), file=sys.stderr, flush=True)
+        return {"workload": "%d jobs, niter=%d, titer=%d, %s only" % (count, niter, titer, part),
+                "value": round(vox / t, 1), "unit": "voxel-iter/s", "wall_s": round(t, 3), "voxel_iterations": vox}
     t0 = time.perf_counter()
     batch.solve_sequential(jobs_a)
     t_seq = time.perf_counter() - t0
@@ -299,6 +323,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=64, help="jobs in the batched / served mixes")
     ap.add_argument("--job-niter", type=int, default=64, help="outer iterations per batched / served job")
     ap.add_argument("--job-titer", type=int, default=4)
+    ap.add_argument("--mix-part", default="both", choices=["both", "sequential", "batched"],
+                    help="--mode batched: run both halves of the comparison, or one of them")
     ap.add_argument("--rate", type=float, default=20.0, help="serving arrival rate (jobs/s)")
     ap.add_argument("--light-rate", type=float, default=0.5,
                     help="arrival rate of the second, below-capacity serving run (jobs/s; 0 skips it)")
@@ -313,7 +339,7 @@ def main():
     if a.mode in ("batched", "served"):
         out = {"metric": METRIC, "n_gpus": 1, "dtype": "f32", "data": "synthetic"}
         if a.mode == "batched":
-            out["batched"] = batched_bench(a.jobs, a.job_niter, a.job_titer)
+            out["batched"] = batched_bench(a.jobs, a.job_niter, a.job_titer, part=a.mix_part)
         else:
             out["served"] = served_bench(a.jobs, a.job_niter, a.job_titer, a.rate)
             if a.light_rate > 0:
@@ -362,16 +388,31 @@ def main():
         if b is not None:
             kern[name]["alg_bytes"] = b
             kern[name]["gbps"] = round(b / (avg * 1e-3) / 1e9, 1)
+        fl = kernel_flops(name, n, nt, m, zb)
+        if fl is not None:
+            kern[name]["alg_flops"] = fl
+            kern[name]["tflops"] = round(fl / (avg * 1e-3) / 1e12, 2)
     total = sum(t for _, t in prof.values())
     for name, (cnt, tot) in prof.items():
         kern[name]["share"] = round(tot / total, 4)
     dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
     db = kernel_bytes(dom, n, nt, m, zb, nzp)
+    dfl = kernel_flops(dom, n, nt, m, zb)
     davg = prof[dom][1] / prof[dom][0] * 1e-3
     achieved = db / davg / 1e9
-    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4),
-            "traffic": measured_traffic(a.config, dom), "alg_bytes_per_launch": db}
+    hbm_view = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved / (HBM_PEAK / 1e9), 4)}
+    # The binding roofline is the larger of bytes / HBM peak and FLOPs / FP32 matrix peak:
+    # the window kernels (about 30 FLOP per algorithmic byte, above the 19.7 FLOP/byte
+    # ridge point of MI355X) are bound by the matrix cores, the rest by HBM.
+    if dfl is not None and dfl / FP32_MFMA_PEAK > db / HBM_PEAK:
+        tf = dfl / davg / 1e12
+        roof = {"kernel": dom, "bound": "mfma", "achieved": round(tf, 2), "peak": FP32_MFMA_PEAK / 1e12,
+                "unit": "TFLOP/s", "frac": round(tf / (FP32_MFMA_PEAK / 1e12), 4), "alg_flops_per_launch": dfl,
+                "hbm_view": hbm_view}
+    else:
+        roof = dict(hbm_view, kernel=dom)
+    roof.update({"traffic": measured_traffic(a.config, dom), "alg_bytes_per_launch": db})
     B = byte_model(n, nt, m, nz, nzp, a.titer)
     Bp = byte_model(n, nt, m, nz, nzp, a.titer, pairs=True)
     # the API default titer = 1 (solver.py:47) on the same solver and data, as a secondary figure
