@@ -46,6 +46,26 @@ __host__ __device__ constexpr int tiles_per_side(int N) { return (N + TS - 1) / 
 
 __device__ __forceinline__ float sgn(int i) { return (i & 1) ? -1.0f : 1.0f; }
 
+// Element loop of the runtime-planned FFT passes: for idx = tid + q*nthr < total,
+// ld(idx) (global loads) for CNT elements at a time, all issued before st(idx, value)
+// uses any of them.  The loads run unconditionally on clamped indices (total > 0), so
+// that the compiler's wait counts stay exact and the loads are in flight together;
+// a loop that waits for every load before the next costs a full memory latency per
+// element.
+template <int CNT, typename V, typename LD, typename ST>
+__device__ __forceinline__ void staged_elems(int tid, int nthr, int total, LD ld, ST st) {
+    for (int base = 0; base < total; base += CNT * nthr) {
+        V tmp[CNT];
+#pragma unroll
+        for (int q = 0; q < CNT; q++) tmp[q] = ld(min(base + tid + q * nthr, total - 1));
+#pragma unroll
+        for (int q = 0; q < CNT; q++) {
+            const int idx = base + tid + q * nthr;
+            if (idx < total) st(idx, tmp[q]);
+        }
+    }
+}
+
 // H holds each (slice, pair) spectrum as groups of g adjacent columns kx, each
 // group [n rows y][g] (ceil(2n/g) groups: for odd n the last group is half used);
 // g = 2n is the plain row-major layout.  The forward
@@ -202,16 +222,22 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__r
         const size_t plane = (size_t)n * n;
         const float *fz = f + (size_t)z * M * plane + (size_t)y * n;
         const float dy = deap[y];
-        for (int idx = tid; idx < Tc * n; idx += nthr) {
-            const int pt = idx / n, x = idx - pt * n;
-            const int i0 = 2 * (p0 + pt);
-            const float va = fz[(size_t)i0 * plane + x];
-            const float vb = i0 + 1 < M ? fz[(size_t)(i0 + 1) * plane + x] : 0.f;
-            const float sc = deap[x] * dy * sgn(x + h);
-            lds[pt * stride + lp(x + h)] = make_float2(va * sc, vb * sc);
-            const int q = x < h ? x : x + n;  // zero padding
-            lds[pt * stride + lp(q)] = make_float2(0.f, 0.f);
-        }
+        staged_elems<8, float2>(
+            tid, nthr, Tc * n,
+            [&](int idx) {
+                const int pt = idx / n, x = idx - pt * n;
+                const int i0 = 2 * (p0 + pt);
+                const float va = fz[(size_t)i0 * plane + x];
+                const float vb = fz[(size_t)min(i0 + 1, M - 1) * plane + x];
+                const float sc = deap[x] * dy * sgn(x + h);
+                return make_float2(va * sc, i0 + 1 < M ? vb * sc : 0.f);
+            },
+            [&](int idx, float2 val) {
+                const int pt = idx / n, x = idx - pt * n;
+                lds[pt * stride + lp(x + h)] = val;
+                const int q = x < h ? x : x + n;  // zero padding
+                lds[pt * stride + lp(q)] = make_float2(0.f, 0.f);
+            });
         __syncthreads();
         fft_lds<FFT_E, false, true, NP == -2>(lds, stride, Tc, plan, tw, tid, nthr);
         for (int idx = tid; idx < Tc * N; idx += nthr) {
@@ -260,19 +286,27 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
         const int kt = L % ntile;
         const int zj = L / ntile;  // z*MP + p
         const int kx0 = kt * T;
-        for (int idx = tid; idx < T * n; idx += nthr) {
-            const int t = idx % T, y = idx / T;
-            lds[t * stride + lp(y + h)] = cscale(H[h_index(zj, n, y, kx0 + t, HG)], sgn(y + h));
-            const int q = y < h ? y : y + n;
-            lds[t * stride + lp(q)] = make_float2(0.f, 0.f);
-        }
+        staged_elems<8, float2>(
+            tid, nthr, T * n,
+            [&](int idx) {
+                const int t = idx % T, y = idx / T;
+                return cscale(H[h_index(zj, n, y, kx0 + t, HG)], sgn(y + h));
+            },
+            [&](int idx, float2 val) {
+                const int t = idx % T, y = idx / T;
+                lds[t * stride + lp(y + h)] = val;
+                const int q = y < h ? y : y + n;
+                lds[t * stride + lp(q)] = make_float2(0.f, 0.f);
+            });
         __syncthreads();
         fft_lds<FFT_E, false, true, NP == -2>(lds, stride, T, plan, tw, tid, nthr);
         float2 *Gs = G + (size_t)zj * N * N + (size_t)kx0 * N;
-        for (int idx = tid; idx < T * N; idx += nthr) {
-            const int t = idx / N, ky = idx - t * N;
-            if (in_run(runs[kx0 + t], ky, N)) Gs[(size_t)t * N + ky] = cscale(lds[t * stride + lp(ky)], sgn(ky));
-        }
+        staged_elems<16, int2>(
+            tid, nthr, T * N, [&](int idx) { return runs[kx0 + idx / N]; },
+            [&](int idx, int2 run) {
+                const int t = idx / N, ky = idx - t * N;
+                if (in_run(run, ky, N)) Gs[(size_t)t * N + ky] = cscale(lds[t * stride + lp(ky)], sgn(ky));
+            });
     }
 }
 
@@ -318,16 +352,22 @@ constexpr int GM_WYP = 17;     // row-tap entries per item (3 zeros, 9 taps, 5 z
 // More than 16 pairs (M > 32) run as one launch per group of at most 8*NB pairs
 // p0 .. p0 + 8*NB - 1 of the MPt pairs: the item sums are linear in the pairs, so
 // launches after the first add their part to gq (accum != 0).
-template <int NB>              // 16-column blocks: pairs of this launch <= 8*NB
+// With few pairs (PK = 2: at most 4, PK = 4: at most 2; NB = 1) PK slices share
+// the 16 columns: slice s of the group holds the real parts of its pairs in
+// columns s*CW + p and the imaginary parts in s*CW + CW/2 + p (CW = 16 / PK), so
+// one pass of matrix-core instructions serves PK slices.
+template <int NB, int PK = 1>  // 16-column blocks: pairs of this launch <= 8*NB; PK slices per pass
 __global__ void __launch_bounds__(GM_THREADS)
 k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
               const int *__restrict__ items, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf,
               int n, int nt, int MPt, int p0, int accum, float beta, int nentries, int nzb, int zg) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
+    static_assert(PK == 1 || (NB == 1 && (PK == 2 || PK == 4)), "slice packing needs one column block");
     constexpr int NC = 16 * NB, HALF = 8 * NB, NCP = NC + 1;
+    constexpr int CW = NC / PK, HP = CW / 2;  // columns per packed slice; its pairs
     constexpr int CELLS = TH * TH;
-    constexpr int NLD = (HALF * CELLS + GM_THREADS - 1) / GM_THREADS;  // tile loads per thread and slice
-    const int MP = min(HALF, MPt - p0);  // pairs of this launch
+    constexpr int NLD = (HALF * CELLS + GM_THREADS - 1) / GM_THREADS;  // tile loads per thread and pass
+    const int MP = min(HP, MPt - p0);  // pairs of this launch
     // 4 spare cells: the last 4-cell step of row TH-1 may read up to 3 cells past
     // the region (their weights are zero; the spare cells are zeroed)
     __shared__ float tile[(CELLS + 4) * NCP];
@@ -349,13 +389,15 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     // per-thread tile byte offsets within one slice's spectra (all pairs of a slice
     // span MP*4n^2*8 < 2^32 bytes); entries outside the region or of absent pairs read
     // the slice's first element and are not stored (the absent pairs' columns of the
-    // tile are zeroed once), so the loads need no predication
+    // tile are zeroed once), so the loads need no predication.  Entry q belongs to
+    // packed slice tsl(q) of the pass.
     unsigned tofs[NLD];
     unsigned keep = 0;  // bit q: entry q is stored
 #pragma unroll
     for (int q = 0; q < NLD; q++) {
         const int idx = tid + q * GM_THREADS;
-        const int p = idx / CELLS, rc = idx - p * CELLS;
+        const int sp = idx / CELLS, rc = idx - sp * CELLS;
+        const int p = sp % HP;
         const int r = rc / TH, c = rc - r * TH;
         int gy = ty0 + r;
         if (gy >= N) gy -= N;
@@ -365,32 +407,39 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
         tofs[q] = ok ? 8u * (unsigned)((p * N + gy) * N + gx) : 0u;
         keep |= ok ? (1u << q) : 0u;
     }
+    auto tsl = [&](int q) { return PK == 1 ? 0 : (tid + q * GM_THREADS) / CELLS / HP; };
     float2 stv[NLD];
-    auto load_tile = [&](int z) {
+    // the tile of the pass starting at slice z, of which nv slices exist (the others
+    // read the last one: finite values in columns whose results are not written)
+    auto load_tile = [&](int z, int nv) {
         const char *Gz = reinterpret_cast<const char *>(G + ((size_t)z * MPt + p0) * N * N);
+        const size_t sb = (size_t)MPt * N * N * sizeof(float2);
 #pragma unroll
-        for (int q = 0; q < NLD; q++) stv[q] = *reinterpret_cast<const float2 *>(Gz + tofs[q]);
+        for (int q = 0; q < NLD; q++)
+            stv[q] = *reinterpret_cast<const float2 *>(Gz + (PK == 1 ? 0 : (size_t)min(tsl(q), nv - 1) * sb) + tofs[q]);
     };
     auto store_tile = [&] {
 #pragma unroll
         for (int q = 0; q < NLD; q++) {
             if (keep & (1u << q)) {
                 const int idx = tid + q * GM_THREADS;
-                const int p = idx / CELLS, rc = idx - p * CELLS;
-                tile[rc * NCP + p] = stv[q].x;
-                tile[rc * NCP + HALF + p] = stv[q].y;
+                const int sp = idx / CELLS, rc = idx - sp * CELLS;
+                const int col = (sp / HP) * CW + sp % HP;
+                tile[rc * NCP + col] = stv[q].x;
+                tile[rc * NCP + HP + col] = stv[q].y;
             }
         }
     };
-    // columns of absent pairs (MP < HALF) stay zero
+    // columns of absent pairs (MP < HP) stay zero
     for (int i = tid; i < CELLS * HALF; i += GM_THREADS) {
-        const int rc = i / HALF, p = i - rc * HALF;
-        if (p >= MP) {
-            tile[rc * NCP + p] = 0.f;
-            tile[rc * NCP + HALF + p] = 0.f;
+        const int rc = i / HALF, sp = i - rc * HALF;
+        if (sp % HP >= MP) {
+            const int col = (sp / HP) * CW + sp % HP;
+            tile[rc * NCP + col] = 0.f;
+            tile[rc * NCP + HP + col] = 0.f;
         }
     }
-    load_tile(z0);
+    load_tile(z0, min(PK, zc));
     // stage the items: 4 threads per item (rows taps 0-4 / 5-8, cell taps 0-4 / 5-8)
     {
         const int it = tid >> 2, part = tid & 3;
@@ -448,7 +497,7 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
         for (int b = 0; b < NB; b++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int pp = (16 * b + 4 * lk + v) % HALF;
+                const int pp = ((16 * b + 4 * lk + v) % CW) % HP;
                 cfv[b][v] = (oid >= 0 && pp < MP) ? crow[pp] : make_float2(0.f, 0.f);
             }
     }
@@ -465,10 +514,10 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     const float *wrow = &wty[b0 + li][3 + max(rlo - myly, -3)];
     const float *tb = tile + (rlo * TH + clo + lk) * NCP + li;
     float2 *out = gq + (size_t)z0 * nt * n * 2;
-    for (int zz = 0; zz < zc; zz++) {
-        if (zz > 0) lds_barrier();  // previous slice's tile fully read
+    for (int zz = 0; zz < zc; zz += PK) {
+        if (zz > 0) lds_barrier();  // previous pass's tile fully read
         store_tile();
-        if (zz + 1 < zc) load_tile(z0 + zz + 1);
+        if (zz + PK < zc) load_tile(z0 + zz + PK, min(PK, zc - zz - PK));
         lds_barrier();
         if (rhi >= 0) {
             v4f acc0[NB], acc1[NB];
@@ -514,15 +563,15 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
                 case 10: rows(std::integral_constant<int, 11>{}); break;
                 default: rows(std::integral_constant<int, 12>{}); break;
             }
-            // lane (li, lk): D[16*b + 4*lk + v][li]; column c is pair c % HALF, imaginary
-            // part for c >= HALF
+            // lane (li, lk): D[16*b + 4*lk + v][li]; column c is pair (c % CW) % HP of
+            // packed slice c / CW, imaginary part for c % CW >= HP
             float sx = 0.f, sy = 0.f;
 #pragma unroll
             for (int b = 0; b < NB; b++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const float d = acc0[b][v] + acc1[b][v];
-                    if (16 * b + 4 * lk + v < HALF) {
+                    if ((16 * b + 4 * lk + v) % CW < HP) {
                         sx += cfv[b][v].x * d;
                         sy += cfv[b][v].y * d;
                     } else {  // cf * (i d)
@@ -530,10 +579,19 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
                         sy += cfv[b][v].x * d;
                     }
                 }
-            sx = add_xor32(add_xor16(sx), bp32);
-            sy = add_xor32(add_xor16(sy), bp32);
-            if (lk == 0 && oid >= 0) {
-                float2 *o = out + (size_t)zz * nt * n * 2 + oid;
+            // the lanes of one packed slice hold its CW columns: PK = 1: lk = 0..3
+            // (xor 16, xor 32), PK = 2: lk = 2s, 2s + 1 (xor 16), PK = 4: lk = s
+            if (PK <= 2) {
+                sx = add_xor16(sx);
+                sy = add_xor16(sy);
+            }
+            if (PK == 1) {
+                sx = add_xor32(sx, bp32);
+                sy = add_xor32(sy, bp32);
+            }
+            const int zs = zz + (PK == 1 ? 0 : lk * PK / 4);  // this lane's slice
+            if ((lk % (4 / PK)) == 0 && oid >= 0 && zs < zc) {
+                float2 *o = out + (size_t)zs * nt * n * 2 + oid;
                 if (accum) {
                     const float2 pv = *o;
                     sx += pv.x;
@@ -617,41 +675,48 @@ k_ray(const float2 *__restrict__ gq, float2 *__restrict__ q, const float *__rest
     const int r0 = blockIdx.x * T;
     const int Tc = min(T, nrays - r0);
     if (mode & RAY_FWD) {
-        for (int idx = tid; idx < Tc * n; idx += nthr) {
-            const int t = idx / n, s = idx - t * n;
-            float2 v = make_float2(0.f, 0.f);
-            if (s != 0) {
+        // the slot pair of s == 0 is never written by the gather: loaded, not used
+        staged_elems<16, float2>(
+            tid, nthr, Tc * n,
+            [&](int idx) {
+                const int t = idx / n, s = idx - t * n;
                 const float4 pq = reinterpret_cast<const float4 *>(gq)[(size_t)(r0 + t) * n + s];
-                v = cmul(make_float2(pq.x + pq.z, pq.y - pq.w), pre_f[s]);
-            }
-            lds[t * stride + lp(s)] = v;
-        }
+                return cmul(make_float2(pq.x + pq.z, pq.y - pq.w), pre_f[s]);
+            },
+            [&](int idx, float2 val) {
+                const int t = idx / n, s = idx - t * n;
+                lds[t * stride + lp(s)] = s != 0 ? val : make_float2(0.f, 0.f);
+            });
         __syncthreads();
         fft_lds<FFT_E, true, true, NP == -2>(lds, stride, Tc, plan, tw, tid, nthr);
-        for (int idx = tid; idx < Tc * n; idx += nthr) {
-            const int t = idx / n, s = idx - t * n;
-            const size_t o = (size_t)(r0 + t) * n + s;
-            float val = lds[t * stride + lp(s)].x * post_f[s];
-            if (mode & RAY_SUBG) val -= g[o];
-            acc += (double)val * val;
-            if (h1out) h1out[o] = val;
-            lds[t * stride + lp(s)] = make_float2(val * pre_a[s], 0.f);
-        }
+        staged_elems<16, float>(
+            tid, nthr, Tc * n,
+            [&](int idx) { return (mode & RAY_SUBG) ? g[(size_t)r0 * n + idx] : 0.f; },
+            [&](int idx, float gv) {
+                const int t = idx / n, s = idx - t * n;
+                const size_t o = (size_t)(r0 + t) * n + s;
+                const float val = lds[t * stride + lp(s)].x * post_f[s] - gv;
+                acc += (double)val * val;
+                if (h1out) h1out[o] = val;
+                lds[t * stride + lp(s)] = make_float2(val * pre_a[s], 0.f);
+            });
     } else {
-        for (int idx = tid; idx < Tc * n; idx += nthr) {
-            const int t = idx / n, s = idx - t * n;
-            lds[t * stride + lp(s)] = make_float2(h1in[(size_t)(r0 + t) * n + s] * pre_a[s], 0.f);
-        }
+        staged_elems<16, float>(
+            tid, nthr, Tc * n, [&](int idx) { return h1in[(size_t)r0 * n + idx]; },
+            [&](int idx, float hv) {
+                const int t = idx / n, s = idx - t * n;
+                lds[t * stride + lp(s)] = make_float2(hv * pre_a[s], 0.f);
+            });
     }
     if (mode & RAY_ADJ) {
         __syncthreads();
         fft_lds<FFT_E, false, true, NP == -2>(lds, stride, Tc, plan, tw, tid, nthr);
-        for (int idx = tid; idx < Tc * n; idx += nthr) {
-            const int t = idx / n, s = idx - t * n;
-            float2 v = make_float2(0.f, 0.f);
-            if (s != 0) v = cmul(lds[t * stride + lp(s)], post_a[s]);
-            q[(size_t)(r0 + t) * n + s] = v;
-        }
+        staged_elems<16, float2>(
+            tid, nthr, Tc * n, [&](int idx) { return post_a[idx % n]; },
+            [&](int idx, float2 pa) {
+                const int t = idx / n, s = idx - t * n;
+                q[(size_t)r0 * n + idx] = s != 0 ? cmul(lds[t * stride + lp(s)], pa) : make_float2(0.f, 0.f);
+            });
     }
     }
     if (partial) {
@@ -723,18 +788,24 @@ __global__ void __launch_bounds__(1024) k_scatter_weights(const AdjItem *__restr
     wtab[((size_t)ci * SM_CH + sm) * 8 + part] = make_float4(w[0], w[1], w[2], w[3]);
 }
 
-template <int NB, int ZW>   // 16-column blocks: MP <= 8*NB (NB = 1 or 2); ZW slices per workgroup
+// With few pairs (PK = 2: at most 4, PK = 4: at most 2; NB = 1) PK slices share the
+// 16 columns of P and of the accumulators: slice zz of the workgroup goes to
+// plane zz / PK, columns (zz % PK)*CW + p (real) and + CW/2 (imaginary), CW = 16 / PK.
+template <int NB, int ZW, int PK = 1>  // 16-column blocks: MP <= 8*NB (NB = 1 or 2); ZW slices per workgroup
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))  // two workgroups per CU
 k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__restrict__ arec,
           const TileEntry *__restrict__ tiles, const int2 *__restrict__ crange,
           const int2 *__restrict__ brange, const unsigned char *__restrict__ blist, const float2 *__restrict__ cf,
           const float4 *__restrict__ wtab, int n, int nt, int MPt, int p0, int ntiles, int nzb) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
+    static_assert(PK == 1 || (NB == 1 && (PK == 2 || PK == 4) && ZW % PK == 0), "slice packing: one column block");
+    constexpr int NG = ZW / PK;        // planes of P and accumulators
+    constexpr int CW = 16 * NB / PK, HP = CW / 2;  // columns per packed slice; its pairs
     // this launch writes the G planes of pairs p0 .. p0 + MP - 1 of the MPt pairs
     // (more than 16 pairs run as one launch per group)
-    const int MP = min(8 * NB, MPt - p0);
+    const int MP = min(HP, MPt - p0);
     constexpr int NC = 16 * NB;
-    constexpr int HALF = 8 * NB;       // column of the imaginary part of pair 0
+    constexpr int HALF = 8 * NB;       // column of the imaginary part of pair 0 (PK = 1)
     // P rows: NC columns, unpadded.  The host orders each block's list in groups of
     // 4 slots that are distinct mod 4 wherever it can, so one matrix-core operand
     // read (4 slots x 16 columns) meets 4 different 16-bank groups; for NB = 2 the
@@ -745,7 +816,7 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
     constexpr int NS = SM_CH + 1;      // slots incl. the zero slot
     __shared__ float wyr[NS][TS];
     __shared__ float wxr[NS][TS];
-    __shared__ float sP[ZW][NS][NCP];
+    __shared__ float sP[NG][NS][NCP];
     __shared__ unsigned char sl[SM_LB];
     const int tid = threadIdx.x;
     const int N = 2 * n, nT = tiles_per_side(N);
@@ -765,12 +836,12 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
         wyr[SM_CH][tid] = 0.f;
         wxr[SM_CH][tid] = 0.f;
     }
-    if (tid < ZW * NCP) sP[tid / NCP][SM_CH][tid % NCP] = 0.f;
-    v4f acc[ZW][NB];
+    if (tid < NG * NCP) sP[tid / NCP][SM_CH][tid % NCP] = 0.f;
+    v4f acc[NG][NB];
 #pragma unroll
-    for (int zz = 0; zz < ZW; zz++)
+    for (int g = 0; g < NG; g++)
 #pragma unroll
-        for (int b = 0; b < NB; b++) acc[zz][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int b = 0; b < NB; b++) acc[g][b] = (v4f){0.f, 0.f, 0.f, 0.f};
     // chunk staging data are prefetched into registers one chunk ahead, so the
     // dependent global loads (item -> q, coefficients) overlap the MFMA loop; the
     // weight rows come precomputed (k_scatter_weights), one float4 per thread
@@ -835,9 +906,14 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
                     const int j = part + 8 * q;
                     // q' * conj(cf); zero for pairs j >= MP
                     const float2 p = j < MP ? cmulc(qv, pph[q]) : make_float2(0.f, 0.f);
-                    const int sw = NB == 2 ? 16 * ((sm >> 1) & 1) : 0;
-                    sP[zz][sm][j ^ sw] = p.x;
-                    sP[zz][sm][(HALF + j) ^ sw] = p.y;
+                    if (PK == 1) {
+                        const int sw = NB == 2 ? 16 * ((sm >> 1) & 1) : 0;
+                        sP[zz][sm][j ^ sw] = p.x;
+                        sP[zz][sm][(HALF + j) ^ sw] = p.y;
+                    } else if (j < HP) {
+                        sP[zz / PK][sm][(zz % PK) * CW + j] = p.x;
+                        sP[zz / PK][sm][(zz % PK) * CW + HP + j] = p.y;
+                    }
                 }
             }
         }
@@ -859,34 +935,37 @@ k_scatter(float2 *__restrict__ G, const float2 *__restrict__ qs, const int2 *__r
             const int sh = NB == 2 ? (slot >> 1) & 1 : 0;  // swapped halves
             const float *pp = &sP[0][slot][li + 16 * sh];
 #pragma unroll
-            for (int zz = 0; zz < ZW; zz++) {
+            for (int g = 0; g < NG; g++) {
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
-                    const float bv = pp[zz * NS * NCP + b * (16 - 32 * sh)];
-                    acc[zz][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[zz][b], 0, 0, 0);
+                    const float bv = pp[g * NS * NCP + b * (16 - 32 * sh)];
+                    acc[g][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[g][b], 0, 0, 0);
                 }
             }
         }
     }
-    // acc[zz][b][v] = D_zz[cell 4*lk + v][col 16*b + li] = block row lk, cell v: real
-    // part of pair p in column p, imaginary part in column HALF + p
+    // acc[g][b][v] = D_g[cell 4*lk + v][col 16*b + li] = block row lk, cell v: the
+    // real part of pair p of packed slice s in column s*CW + p, its imaginary part in
+    // column s*CW + HP + p (PK = 1: s = 0, HP = HALF)
     float2 *Gt = G + (size_t)(ty0 + br0 + lk) * N + tx0 + bc0;
+    const int ls = PK == 1 ? 0 : li / CW, lp_ = PK == 1 ? li : li % CW;  // this lane's packed slice, pair
 #pragma unroll
-    for (int zz = 0; zz < ZW; zz++) {
+    for (int g = 0; g < NG; g++) {
         float re[4], im[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             if constexpr (NB == 1) {
-                re[v] = acc[zz][0][v];
-                im[v] = __shfl(acc[zz][0][v], (l & 48) | ((li + 8) & 15), 64);
+                re[v] = acc[g][0][v];
+                im[v] = __shfl(acc[g][0][v], (l & 48) | ((li + HP) & 15), 64);
             } else {
-                re[v] = acc[zz][0][v];
-                im[v] = acc[zz][1][v];
+                re[v] = acc[g][0][v];
+                im[v] = acc[g][1][v];
             }
         }
+        const int zz = g * PK + ls;
         // cells past 2n (a narrow last tile) are not this tile's; 2n and tx0 + bc0 are even
-        if (zz < zc && li < MP && ty0 + br0 + lk < N) {
-            float4 *o = reinterpret_cast<float4 *>(Gt + ((size_t)(z0 + zz) * MPt + p0 + li) * N * N);
+        if (zz < zc && lp_ < MP && ty0 + br0 + lk < N) {
+            float4 *o = reinterpret_cast<float4 *>(Gt + ((size_t)(z0 + zz) * MPt + p0 + lp_) * N * N);
             if (tx0 + bc0 < N) o[0] = make_float4(re[0], im[0], re[1], im[1]);
             if (tx0 + bc0 + 2 < N) o[1] = make_float4(re[2], im[2], re[3], im[3]);
         }
@@ -935,11 +1014,17 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
         const int zj = L / ntile;
         const int kx0 = kt * T;
         const float2 *Gs = G + (size_t)zj * N * N + (size_t)kx0 * N;
-        for (int idx = tid; idx < T * N; idx += nthr) {
-            const int t = idx / N, ky = idx - t * N;
-            lds[t * stride + lp(ky)] =
-                in_run(runs[kx0 + t], ky, N) ? cscale(Gs[(size_t)t * N + ky], sgn(ky)) : make_float2(0.f, 0.f);
-        }
+        staged_elems<16, float2>(
+            tid, nthr, T * N,
+            [&](int idx) {
+                const int t = idx / N, ky = idx - t * N;
+                const float2 g = Gs[(size_t)t * N + ky];  // finite: cells outside the runs hold zero
+                return in_run(runs[kx0 + t], ky, N) ? cscale(g, sgn(ky)) : make_float2(0.f, 0.f);
+            },
+            [&](int idx, float2 val) {
+                const int t = idx / N, ky = idx - t * N;
+                lds[t * stride + lp(ky)] = val;
+            });
         __syncthreads();
         fft_lds<FFT_E, true, true, NP == -2>(lds, stride, T, plan, tw, tid, nthr);
         const int ga = h_group_adj(N);
@@ -1010,18 +1095,29 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
         const int y0 = yt * T;
         const int Tc = min(T, n - y0);
         const int ga = h_group_adj(N);
-        for (int idx = tid; idx < Tc * N; idx += nthr) {
-            const int t = idx / N, kx = idx - t * N;
-            lds[t * stride + lp(kx)] = cscale(H[h_index(zj, n, y0 + t, kx, ga)], sgn(kx));
-        }
+        staged_elems<16, float2>(
+            tid, nthr, Tc * N,
+            [&](int idx) {
+                const int t = idx / N, kx = idx - t * N;
+                return cscale(H[h_index(zj, n, y0 + t, kx, ga)], sgn(kx));
+            },
+            [&](int idx, float2 val) {
+                const int t = idx / N, kx = idx - t * N;
+                lds[t * stride + lp(kx)] = val;
+            });
         __syncthreads();
         fft_lds<FFT_E, true, true, NP == -2>(lds, stride, Tc, plan, tw, tid, nthr);
         float2 *vs = v + ((size_t)zj * n + y0) * n;
-        for (int idx = tid; idx < Tc * n; idx += nthr) {
-            const int t = idx / n, x = idx - t * n;
-            const float s = sgn(x + h) * deap[x] * deap[y0 + t];
-            vs[(size_t)t * n + x] = cscale(lds[t * stride + lp(x + h)], s);
-        }
+        staged_elems<8, float>(
+            tid, nthr, Tc * n,
+            [&](int idx) {
+                const int t = idx / n, x = idx - t * n;
+                return sgn(x + h) * deap[x] * deap[y0 + t];
+            },
+            [&](int idx, float sc) {
+                const int t = idx / n, x = idx - t * n;
+                vs[(size_t)t * n + x] = cscale(lds[t * stride + lp(x + h)], sc);
+            });
     }
 }
 

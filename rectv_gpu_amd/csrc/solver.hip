@@ -1020,13 +1020,17 @@ struct Solver {
         const int pg = gather_group();
         for (int p0 = 0; p0 < MP; p0 += pg) {
             const int np = std::min(pg, MP - p0), acc = p0 > 0 ? 1 : 0;
+            // few pairs: 2 or 4 slices share the 16 matrix-core columns
+            const int pk = MP <= 2 ? 4 : MP <= 4 ? 2 : 1;
             launch("gather", [&] {
-                if (np <= 8)
-                    hipLaunchKernelGGL(k_gather_mfma<1>, dim3(blocks), dim3(GM_THREADS), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, p0, acc, beta, nfm_entries, zc, zg);
-                else
-                    hipLaunchKernelGGL(k_gather_mfma<2>, dim3(blocks), dim3(GM_THREADS), 0, st, d_G.p, d_gq.p, d_geo.p,
-                                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, p0, acc, beta, nfm_entries, zc, zg);
+#define RTV_GATHER(NB_, PK_)                                                                                          \
+    hipLaunchKernelGGL((k_gather_mfma<NB_, PK_>), dim3(blocks), dim3(GM_THREADS), 0, st, d_G.p, d_gq.p, d_geo.p,      \
+                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, p0, acc, beta, nfm_entries, zc, zg)
+                if (pk == 4) RTV_GATHER(1, 4);
+                else if (pk == 2) RTV_GATHER(1, 2);
+                else if (np <= 8) RTV_GATHER(1, 1);
+                else RTV_GATHER(2, 1);
+#undef RTV_GATHER
             });
         }
     }
@@ -1037,15 +1041,17 @@ struct Solver {
         const int blocks = na_tiles * ((zc + ZW - 1) / ZW);
         for (int p0 = 0; p0 < MP; p0 += 16) {
             const int np = std::min(16, MP - p0);
+            // few pairs: 2 or 4 of the workgroup's slices share the 16 matrix-core columns
+            const int pk = (MP <= 2 && ZW >= 4) ? 4 : MP <= 4 ? 2 : 1;
             launch("scatter", [&] {
-                if (np <= 8)
-                    hipLaunchKernelGGL((k_scatter<1, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
-                                       d_atiles.p, d_crange.p, d_brange.p, d_blist.p, cf, d_wtab.p, n, nt, MP, p0,
-                                       na_tiles, zc);
-                else
-                    hipLaunchKernelGGL((k_scatter<2, ZW>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,
-                                       d_atiles.p, d_crange.p, d_brange.p, d_blist.p, cf, d_wtab.p, n, nt, MP, p0,
-                                       na_tiles, zc);
+#define RTV_SCATTER(NB_, PK_)                                                                                   \
+    hipLaunchKernelGGL((k_scatter<NB_, ZW, PK_>), dim3(blocks), dim3(1024), 0, st, d_G.p, d_q.p, d_arec.p,      \
+                       d_atiles.p, d_crange.p, d_brange.p, d_blist.p, cf, d_wtab.p, n, nt, MP, p0, na_tiles, zc)
+                if (pk == 4) RTV_SCATTER(1, (ZW >= 4 ? 4 : 1));
+                else if (pk == 2) RTV_SCATTER(1, 2);
+                else if (np <= 8) RTV_SCATTER(1, 1);
+                else RTV_SCATTER(2, 1);
+#undef RTV_SCATTER
             });
         }
     }

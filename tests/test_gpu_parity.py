@@ -121,6 +121,14 @@ def _case(name):
         n, nt, m, nz = 56, 15, 3, 2
         th = np.sort(rng.uniform(0, 3 * np.pi, nt)).astype(np.float32)
         phi, center = B.takephi(nt, m), n / 2
+    elif name == "n48_m4_packed_partial":  # 2 pairs: 4 slices share the matrix-core columns; 6 = 4 + 2 slices
+        n, nt, m, nz = 48, 20, 4, 6
+        th = B.angles(10, 2)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
+    elif name == "n40_m8_packed_partial":  # 4 pairs: 2 slices share the columns; 3 = 2 + 1 slices
+        n, nt, m, nz = 40, 16, 8, 3
+        th = np.sort(rng.uniform(0, 4 * np.pi, nt)).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2 - 0.5
     else:
         raise KeyError(name)
     return n, nt, m, nz, th, phi, center, rng
@@ -130,7 +138,7 @@ CASES = ["n64_rand", "n72_mixed_radix_m3", "n384_mixed_radix", "cfg_small", "n64
          "n1024_column_rounds", "n2048_column_rounds", "n480_radix5", "n504_radix7", "n40_radix5_small",
          "n56_radix7_small", "n64_m40_pair_groups", "n32_m65_pair_groups", "n36_narrow_tile",
          "n50_three_tiles", "n100_narrow_tile", "n44_radix11", "n34_radix17", "n136_radix17", "n45_odd",
-         "n17_odd_prime", "n63_odd"]
+         "n17_odd_prime", "n63_odd", "n48_m4_packed_partial", "n40_m8_packed_partial"]
 
 
 @pytest.mark.parametrize("name", CASES)

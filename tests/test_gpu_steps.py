@@ -109,6 +109,37 @@ def test_prox_phase_parity(gpu, n, nt, m, nz, nzp, rho, lam1):
     np.testing.assert_array_equal(st["f"], A)  # the previous iterate is kept
 
 
+def test_prox_phase_parity_rho_halved(gpu):
+    """The closing phase when the rule halves rho (s > 10 r: a small threshold leaves
+    psi close to grad fcur + mu/rho while fcur moved far from the previous iterate):
+    div(mu - rho' psi) is then formed from chi and mu with rho' = rho / 2."""
+    from rectv_gpu_amd import Solver
+    n, nt, m, nz, nzp, rho, lam1 = 40, 20, 4, 4, 2, 2.0, 1.0
+    th, phi, A, fcur, psi, mu, g = _state(7, n, nt, m, nz, nzp)
+    ops = O.Operators(n, nt, m, th, phi, n / 2, lam1)
+    fe, fl, fr = OS.chunk_ext(fcur.astype(np.float64), A, 0, nzp, nz // nzp)
+    v = ops.gradient(fe, fl, fr) + mu[0] / rho
+    lam0 = float(1e-3 * rho * np.median(np.linalg.norm(v, axis=-1)))
+    say("case closing phase, rho halved", n, nt, m, nz, nzp, rho, lam0, lam1)
+    with Solver(n, nt, m, nz, nzp, 1) as s:
+        s.set_inputs(g, th, phi)
+        s.set_state(A, psi, mu, rho, lam1)
+        row = s.apply_prox(fcur, lam0, lam1)
+        st = s.get_state()
+    ref = OS.prox_step(ops, A.astype(np.float64), fcur.astype(np.float64), psi.astype(np.float64),
+                       mu.astype(np.float64), rho, lam0, lam1, nzp)
+    assert ref["rho_next"] == rho / 2, ref["rho_next"]
+    errs = {k: rel(st[k], ref[k]) for k in ("psi", "dchi")}
+    # mu' = rho (p - psi) with p = grad fcur + mu/rho is a small difference here
+    # (|mu'| <= lambda0 per group, far below |mu| + rho |grad fcur|), so its error is
+    # measured against the scale of the dual it was formed from
+    errs["mu"] = float(np.linalg.norm(st["mu"] - ref["mu"]) / np.linalg.norm(mu))
+    errs.update({k: abs(row[c] - ref[k]) / abs(ref[k]) for c, k in ((0, "r"), (1, "s"))})
+    say("  ", {k: "%.2e" % v for k, v in errs.items()}, "rho", row[2], "->", st["rho"])
+    assert all(v < TOL for v in errs.values()), errs
+    assert st["rho"] == rho / 2
+
+
 def test_state_round_trip_and_resume(gpu):
     """A run of 2 + 3 outer iterations with the state saved after 2 and restored into
     a fresh solver equals 5 iterations in one go (1e-6).  The solver keeps
