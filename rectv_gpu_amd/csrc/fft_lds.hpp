@@ -20,6 +20,7 @@ struct FftPlan {
     int nst;
     int radix[FFT_MAX_STAGES];
     int tw_off;  // offset of this size's twiddle table in the shared table buffer
+    int tw_sub;  // lengths R0 * P (R0 = 3, 5; P a power of two): offset of the P-point table
 };
 
 // LDS position of element i of a transform row: one pad slot per 16 elements
@@ -434,6 +435,67 @@ __device__ __forceinline__ void fft_p2_reg(float2 (&v)[16], float2 *row, bool ac
         for (int b = 0; b < MB; b++)
 #pragma unroll
             for (int r = 0; r < RL; r++) v[b + MB * r] = u[b][r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compile-time lengths N = R0 * P (R0 = 3 or 5, P a power of two >= 64): R0
+// sub-groups of TPS = P/16 threads (N/16 threads per transform, as for a power of
+// two).  Thread (g, jl) supplies x[R0*(jl + m*TPS) + g] in v[m] and receives
+// X[jl + (16 g + m)*TPS] in v[m], m = 0..15:
+//   Y_g = P-point DFT of x[R0 j + g] (fft_p2_reg on the sub-row g),
+//   X[k + P q] = sum_g W_N^(g k) w_R0^(g q) Y_g[k]  (k = jl + m*TPS, q = g),
+// the twiddled Y_g exchanged through the transform's LDS row (R0 sub-rows of
+// pitch lp_stride(P); the exchange uses positions lp(g*P + k) of the same row).
+__host__ __device__ constexpr int rp_radix(int N) {
+    return (N % 3 == 0 && ((N / 3) & (N / 3 - 1)) == 0 && N / 3 >= 64) ? 3
+           : (N % 5 == 0 && ((N / 5) & (N / 5 - 1)) == 0 && N / 5 >= 64) ? 5 : 0;
+}
+// LDS row pitch (float2) of one transform of length N on either path
+__host__ __device__ inline int fft_row_pitch(int N) {
+    const int r0 = rp_radix(N);
+    const int a = lp_stride(N);
+    const int b = r0 ? r0 * lp_stride(N / r0) : 0;
+    return a > b ? a : b;
+}
+template <int R0, int P, bool INV>
+__device__ __forceinline__ void fft_rp_reg(float2 (&v)[16], float2 *row, bool active, int g, int jl,
+                                           const float2 *__restrict__ twP, const float2 *__restrict__ twN) {
+    constexpr int TPS = P / 16;
+    const int sub = lp_stride(P);
+    fft_p2_reg<P, INV>(v, row + g * sub, active, jl, twP);
+    __syncthreads();  // the sub-rows become the exchange row
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const int k = jl + m * TPS;
+            float2 w = twN[g * k];
+            if (INV) w.y = -w.y;
+            row[lp(g * P + k)] = cmul(v[m], w);
+        }
+    }
+    __syncthreads();
+    if (active) {
+        // w_R0^(g' g): the row g of the R0-point DFT matrix
+        float wc[R0], ws[R0];
+#pragma unroll
+        for (int gg = 0; gg < R0; gg++) {
+            const int e = (gg * g) % R0;
+            wc[gg] = odd_cos<R0>(e);
+            ws[gg] = INV ? odd_sin<R0>(e) : -odd_sin<R0>(e);
+        }
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const int k = jl + m * TPS;
+            float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int gg = 0; gg < R0; gg++) {
+                const float2 y = row[lp(gg * P + k)];
+                acc.x += y.x * wc[gg] - y.y * ws[gg];
+                acc.y += y.x * ws[gg] + y.y * wc[gg];
+            }
+            v[m] = acc;
+        }
     }
 }
 

@@ -112,8 +112,12 @@ constexpr int RAY_WG = RTV_RAY_WG;  // threads of the per-ray transform workgrou
 // FFT_WG threads, which lets it hold a stage's butterflies in registers without
 // spilling (up to 256 VGPRs); NP = -1 is the same path for transforms that need
 // more than FFT_WG threads (length > 16 FFT_WG).
-__host__ __device__ constexpr int fft_bound(int NP) { return NP == 0 || NP == -2 ? FFT_WG : FFT_THREADS; }
-// NP = -2: runtime plan with a prime factor above 7 (direct stages), at most FFT_WG threads
+__host__ __device__ constexpr int fft_bound(int NP) {
+    return NP == 0 || NP == -2 || (NP > 0 && rp_radix(NP) > 0) ? FFT_WG : FFT_THREADS;
+}
+// NP = -2: runtime plan with a prime factor above 7 (direct stages), at most FFT_WG threads;
+// compile-time lengths R0 * 2^k run at most 384 threads (fft_rp_count) under the FFT_WG bound,
+// which leaves them the registers of the radix-R0 combine
 __host__ __device__ constexpr int fft_rows_p2(int L, int wg = FFT_WG) {
     return (wg / (L / FFT_E)) < 1 ? 1 : ((wg / (L / FFT_E)) < 16 ? (wg / (L / FFT_E)) : 16);
 }
@@ -129,6 +133,15 @@ __host__ __device__ constexpr int fft_rows_p2(int L, int wg = FFT_WG) {
 #endif
 __host__ __device__ constexpr int fft_cols_p2(int L, int threads) {
     return (threads / (L / FFT_E)) < 1 ? 1 : ((threads / (L / FFT_E)) < 16 ? (threads / (L / FFT_E)) : 16);
+}
+// Lengths L = R0 * 2^k (rp_radix(L) != 0): transforms per workgroup, a power of
+// two (it divides L and L/2), at most 16, and enough for whole waves (t * L/16 a
+// multiple of 64 threads); beyond that as many as fit in `threads` threads.
+__host__ __device__ constexpr int fft_rp_count(int L, int threads) {
+    int t = 1;
+    while (t < 16 && (t * (L / FFT_E)) % 64 != 0) t *= 2;
+    while (2 * t <= 16 && 2 * t * (L / FFT_E) <= threads) t *= 2;
+    return t;
 }
 
 // Cells of G inside some gather window, per row kx of G: the shortest circular
@@ -178,7 +191,44 @@ k_fwd_rows(const float *__restrict__ f, float2 *__restrict__ H, const float *__r
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    if constexpr (NP > 0) {
+    if constexpr (NP > 0 && rp_radix(NP) > 0) {
+        // length R0 * P: thread (t, g, jl) of image row yt*TT + t supplies the pixels
+        // x = R0 (jl + m TPS) + g at positions x + n/2 (m = 4..11 of fft_rp_reg) and
+        // receives kx = jl + (16 g + m) TPS
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, R0 = rp_radix(NP), P = NP / R0, TPS = P / 16;
+        constexpr int TT = fft_rp_count(NP, FFT_WG);
+        constexpr int nyt = nn / TT;
+        const int yt = L % nyt;
+        const int zp = L / nyt;  // z*MP + p
+        const int z = zp / MP, p = zp - z * MP;
+        const int t = tid / TPT, r = tid % TPT, g = r / TPS, jl = r % TPS;
+        const int y = yt * TT + t;
+        const size_t plane = (size_t)nn * nn;
+        const float sgi = sgn(jl + g), sgo = sgn(jl);  // (-1)^position of inputs / outputs
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = make_float2(0.f, 0.f);
+        {
+            const int i0 = 2 * p;
+            const float *fa = f + ((size_t)z * M + i0) * plane + (size_t)y * nn;
+            const float sy = deap[y] * sgi;
+            const float *fb = fa + (i0 + 1 < M ? plane : 0);
+            float va[8], vb[8], dx[8];
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const int x = R0 * (jl + m * TPS) + g;
+                va[m] = fa[x];
+                vb[m] = fb[x];
+                dx[m] = deap[x];
+            }
+#pragma unroll
+            for (int m = 0; m < 8; m++)
+                v[m + 4] = make_float2(va[m] * dx[m] * sy, i0 + 1 < M ? vb[m] * dx[m] * sy : 0.f);
+        }
+        fft_rp_reg<R0, P, false>(v, lds + t * stride, true, g, jl, tw + plan.tw_sub, tw + plan.tw_off);
+#pragma unroll
+        for (int m = 0; m < 16; m++) H[h_row_off<TPS, HG>(zp, nn, y, jl, 16 * g + m)] = cscale(v[m], sgo);
+    } else if constexpr (NP > 0) {
         // thread (t, jl): image row yt*TT + t; element positions jl + m*TPT of
         // the 2n-point row; the n image pixels sit at positions 4*TPT .. 12*TPT - 1
         constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_rows_p2(NP);
@@ -257,7 +307,31 @@ k_fwd_cols(const float2 *__restrict__ H, float2 *__restrict__ G, FftPlan plan, c
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    if constexpr (NP > 0) {
+    if constexpr (NP > 0 && rp_radix(NP) > 0) {
+        // length R0 * P: thread (t, g, jl), t fastest, supplies the rows
+        // y = R0 (jl + m TPS) + g and receives ky = jl + (16 g + m) TPS
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, R0 = rp_radix(NP), P = NP / R0, TPS = P / 16;
+        constexpr int TT = fft_rp_count(NP, RTV_FCOLS_THREADS);
+        const int ntile = N / TT;
+        const int kt = L % ntile;
+        const int zj = L / ntile;  // z*MP + p
+        const int t = tid % TT, r = tid / TT, g = r / TPS, jl = r % TPS;
+        const int kx = kt * TT + t;
+        const float sgi = sgn(jl + g), sgo = sgn(jl);
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m + 4] = cscale(H[h_index(zj, nn, R0 * (jl + m * TPS) + g, kx, HG)], sgi);
+        fft_rp_reg<R0, P, false>(v, lds + t * stride, true, g, jl, tw + plan.tw_sub, tw + plan.tw_off);
+        const int2 run = runs[kx];
+        float2 *Gs = G + (size_t)zj * N * N + (size_t)kx * N + jl;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const int ky = jl + (16 * g + m) * TPS;
+            if (in_run(run, ky, N)) st_nt(Gs + (16 * g + m) * TPS, cscale(v[m], sgo));
+        }
+    } else if constexpr (NP > 0) {
         // thread (t, jl), t fastest: adjacent lanes read adjacent columns; the n
         // input rows sit at positions 4*TPT .. 12*TPT - 1
         constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_cols_p2(NP, RTV_FCOLS_THREADS);
@@ -981,7 +1055,36 @@ k_adj_cols(const float2 *__restrict__ G, float2 *__restrict__ H, FftPlan plan, c
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    if constexpr (NP > 0) {
+    if constexpr (NP > 0 && rp_radix(NP) > 0) {
+        // length R0 * P: thread (t, g, jl), t fastest, supplies ky = R0 (jl + m TPS) + g
+        // and receives the positions jl + (16 g + m) TPS, of which the n centre ones
+        // (16 g + m in [4 R0, 12 R0)) are kept as rows y of H
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, R0 = rp_radix(NP), P = NP / R0, TPS = P / 16;
+        constexpr int TT = fft_rp_count(NP, RTV_ACOLS_THREADS);
+        const int ntile = N / TT;
+        const int kt = L % ntile;
+        const int zj = L / ntile;
+        const int t = tid % TT, r = tid / TT, g = r / TPS, jl = r % TPS;
+        const int kx = kt * TT + t;
+        const float sgi = sgn(jl + g), sgo = sgn(jl);
+        const int2 run = runs[kx];
+        const float2 *Gs = G + (size_t)zj * N * N + (size_t)kx * N;
+        const float2 *G0 = G + (size_t)zj * N * N;
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const int ky = R0 * (jl + m * TPS) + g;
+            const bool in = in_run(run, ky, N);
+            v[m] = cscale(*(in ? Gs + ky : G0), in ? sgi : 0.f);
+        }
+        fft_rp_reg<R0, P, true>(v, lds + t * stride, true, g, jl, tw + plan.tw_sub, tw + plan.tw_off);
+        constexpr int GA = h_group_adj(NP);
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const int q = 16 * g + m - 4 * R0;  // centre row index / TPS
+            if (q >= 0 && q < 8 * R0) H[h_index(zj, nn, jl + q * TPS, kx, GA)] = cscale(v[m], sgo);
+        }
+    } else if constexpr (NP > 0) {
         // thread (t, jl), t fastest; output positions 4*TPT .. 12*TPT - 1 are kept
         constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_cols_p2(NP, RTV_ACOLS_THREADS);
         const int ntile = N / TT;
@@ -1044,7 +1147,47 @@ k_adj_rows(const float2 *__restrict__ H, float2 *__restrict__ v, const float *__
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    if constexpr (NP > 0) {
+    if constexpr (NP > 0 && rp_radix(NP) > 0) {
+        // length R0 * P: the workgroup's TT rows of H are read whole into the LDS row
+        // buffers; thread (t, g, jl) takes kx = R0 (jl + m TPS) + g and receives the
+        // positions jl + (16 g + m) TPS, of which the n centre ones are the pixels
+        constexpr int N = NP, nn = NP / 2, TPT = NP / 16, R0 = rp_radix(NP), P = NP / R0, TPS = P / 16;
+        constexpr int TT = fft_rp_count(NP, FFT_WG);
+        constexpr int nyt = nn / TT;
+        const int yt = L % nyt;
+        const int zj = L / nyt;
+        const int t = tid / TPT, r = tid % TPT, g = r / TPS, jl = r % TPS;
+        const int y = yt * TT + t;
+        const float sgi = sgn(jl + g), sgo = sgn(jl);
+        constexpr int GA = h_group_adj(NP);
+        const int y0 = yt * TT;
+        staged_elems<16, float2>(
+            tid, TT * TPT, TT * N,
+            [&](int idx) {
+                const int rr = idx / N, kx = idx - rr * N;
+                return H[h_index(zj, nn, y0 + rr, kx, GA)];
+            },
+            [&](int idx, float2 val) {
+                const int rr = idx / N, kx = idx - rr * N;
+                lds[rr * stride + lp(kx)] = val;
+            });
+        __syncthreads();
+        float2 u[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) u[m] = cscale(lds[t * stride + lp(R0 * (jl + m * TPS) + g)], sgi);
+        __syncthreads();
+        fft_rp_reg<R0, P, true>(u, lds + t * stride, true, g, jl, tw + plan.tw_sub, tw + plan.tw_off);
+        const float sy = sgo * deap[y];
+        float2 *vr = v + ((size_t)zj * nn + y) * nn;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const int q = 16 * g + m - 4 * R0;
+            if (q >= 0 && q < 8 * R0) {
+                const int x = jl + q * TPS;
+                st_nt(vr + x, cscale(u[m], sy * deap[x]));
+            }
+        }
+    } else if constexpr (NP > 0) {
         constexpr int N = NP, nn = NP / 2, TPT = NP / 16, TT = fft_rows_p2(NP);
         constexpr int nyt = nn / TT;
         const int yt = L % nyt;

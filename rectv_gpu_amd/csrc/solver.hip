@@ -46,6 +46,17 @@ static bool has_big_prime(int len) {
             case 1024: return K<1024>;                                                             \
             case 2048: return K<2048>;                                                             \
             case 4096: return K<4096>;                                                             \
+            case 192: return K<192>;                                                               \
+            case 384: return K<384>;                                                               \
+            case 768: return K<768>;                                                               \
+            case 1536: return K<1536>;                                                             \
+            case 3072: return K<3072>;                                                             \
+            case 6144: return K<6144>;                                                             \
+            case 320: return K<320>;                                                               \
+            case 640: return K<640>;                                                               \
+            case 1280: return K<1280>;                                                             \
+            case 2560: return K<2560>;                                                             \
+            case 5120: return K<5120>;                                                             \
             default: return has_big_prime(len) ? K<-2> : fft_tpt(len, FFT_E) > FFT_WG ? K<-1> : K<0>;   \
         }                                                                                          \
     }
@@ -53,8 +64,28 @@ RTV_PICKER(pick_fwd_rows, k_fwd_rows)
 RTV_PICKER(pick_fwd_cols, k_fwd_cols)
 RTV_PICKER(pick_adj_cols, k_adj_cols)
 RTV_PICKER(pick_adj_rows, k_adj_rows)
-RTV_PICKER(pick_ray, k_ray)
 #undef RTV_PICKER
+// per-ray n-point transforms: powers of two at compile time, other lengths planned at run time
+static decltype(&k_ray<0>) pick_ray(int len) {
+    switch (len) {
+        case 64: return k_ray<64>;
+        case 128: return k_ray<128>;
+        case 256: return k_ray<256>;
+        case 512: return k_ray<512>;
+        case 1024: return k_ray<1024>;
+        case 2048: return k_ray<2048>;
+        case 4096: return k_ray<4096>;
+        default: return has_big_prime(len) ? k_ray<-2> : fft_tpt(len, FFT_E) > FFT_WG ? k_ray<-1> : k_ray<0>;
+    }
+}
+// 2n-point passes with a compile-time kernel of length R0 * 2^k (R0 = 3, 5): the lengths above
+static bool rp_kernel(int len) {
+    switch (len) {
+        case 192: case 384: case 768: case 1536: case 3072: case 6144:
+        case 320: case 640: case 1280: case 2560: case 5120: return true;
+        default: return false;
+    }
+}
 
 // ---------------------------------------------------------------- FFT plans
 static bool make_plan(int N, FftPlan &p) {
@@ -579,6 +610,7 @@ struct Solver {
     FftPlan plan_n{}, plan_N{};
     bool odd_n = false, odd_N = false;  // lengths with factors 3, 5, 7
     bool p2N = false;                   // 2n has a compile-time power-of-two kernel
+    bool rpN = false;                   // 2n has a compile-time kernel of length 3 * 2^k or 5 * 2^k
     DBuf<float2> d_tw;
     int T_rows = 1, T_cols = 1, T_fcols = 1, T_acols = 1, T_ray = 1, T_arows = 1, st_n = 0, st_N = 0;
     // state
@@ -822,9 +854,16 @@ struct Solver {
         };
         add(plan_n);
         add(plan_N);
+        rpN = rp_kernel(N);
+        if (rpN) {  // the P-point table of the length-R0 * P kernels
+            FftPlan sub{};
+            sub.n = N / rp_radix(N);
+            add(sub);
+            plan_N.tw_sub = sub.tw_off;
+        }
         upload(d_tw, tw, bytes, st);
         st_n = lp_stride(n);
-        st_N = lp_stride(N);
+        st_N = fft_row_pitch(N);
         // transforms per workgroup: bounded by LDS and by FFT_THREADS threads
         auto tmax = [&](int len, int stride) {
             return std::max(1, std::min(std::max(1, FFT_WG / fft_tpt(len, FFT_E)), lds_max / (8 * stride)));
@@ -841,6 +880,14 @@ struct Solver {
         T_ray = std::min(tn, 16);
         // power-of-two lengths use compile-time row counts inside the kernels
         p2N = (N & (N - 1)) == 0 && N >= 64 && N <= 4096;
+        if (rpN) {
+            T_arows = T_rows = fft_rp_count(N, FFT_WG);
+            T_fcols = fft_rp_count(N, RTV_FCOLS_THREADS);
+            T_acols = fft_rp_count(N, RTV_ACOLS_THREADS);
+            T_cols = std::max(T_fcols, T_acols);
+            RTV_REQUIRE(8 * st_N * T_cols <= lds_max && 8 * st_N * T_rows <= lds_max,
+                        "LDS too small for the 2n-point transforms");
+        }
         if (p2N) {
             T_arows = T_rows = fft_rows_p2(N);
             T_fcols = fft_cols_p2(N, RTV_FCOLS_THREADS);
@@ -987,7 +1034,7 @@ struct Solver {
         (void)S;
         {
             // power-of-two lengths: (slice, pair, T_rows image rows); else (slice, row, T_rows pairs)
-            const int blocks = p2N ? zc * MP * (n / T_rows) : zc * n * ((MP + T_rows - 1) / T_rows);
+            const int blocks = (p2N || rpN) ? zc * MP * (n / T_rows) : zc * n * ((MP + T_rows - 1) / T_rows);
             const int thr = fft_threads(T_rows, N);
             const size_t lds = (size_t)T_rows * st_N * sizeof(float2);
             launch("fwd_rows", [&] {

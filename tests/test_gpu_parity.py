@@ -129,6 +129,22 @@ def _case(name):
         n, nt, m, nz = 40, 16, 8, 3
         th = np.sort(rng.uniform(0, 4 * np.pi, nt)).astype(np.float32)
         phi, center = B.takephi(nt, m), n / 2 - 0.5
+    elif name == "n96_rp3":                # 2n = 192 = 3 * 64: compile-time length 3 * 2^k
+        n, nt, m, nz = 96, 24, 5, 3
+        th = B.angles(12, 2)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
+    elif name == "n160_rp5":               # 2n = 320 = 5 * 64: compile-time length 5 * 2^k
+        n, nt, m, nz = 160, 20, 4, 2
+        th = np.sort(rng.uniform(0, 4 * np.pi, nt)).astype(np.float32)
+        phi, center = B.takephi(nt, m), n / 2 - 0.5
+    elif name == "n320_rp5":               # 2n = 640 = 5 * 128
+        n, nt, m, nz = 320, 12, 6, 1
+        th = B.angles(6, 2)
+        phi, center = B.takephi(nt, m), n / 2 + 1.0
+    elif name == "n768_rp3":               # 2n = 1536 = 3 * 512 (a 1536-pixel detector doubled)
+        n, nt, m, nz = 768, 8, 3, 1
+        th = B.angles(4, 2)
+        phi, center = B.takephi(nt, m), n / 2 - 2.5
     else:
         raise KeyError(name)
     return n, nt, m, nz, th, phi, center, rng
@@ -138,7 +154,8 @@ CASES = ["n64_rand", "n72_mixed_radix_m3", "n384_mixed_radix", "cfg_small", "n64
          "n1024_column_rounds", "n2048_column_rounds", "n480_radix5", "n504_radix7", "n40_radix5_small",
          "n56_radix7_small", "n64_m40_pair_groups", "n32_m65_pair_groups", "n36_narrow_tile",
          "n50_three_tiles", "n100_narrow_tile", "n44_radix11", "n34_radix17", "n136_radix17", "n45_odd",
-         "n17_odd_prime", "n63_odd", "n48_m4_packed_partial", "n40_m8_packed_partial"]
+         "n17_odd_prime", "n63_odd", "n48_m4_packed_partial", "n40_m8_packed_partial", "n96_rp3",
+         "n160_rp5", "n320_rp5", "n768_rp3"]
 
 
 @pytest.mark.parametrize("name", CASES)
