@@ -317,6 +317,8 @@ def _run_case(n, nz, nzp, nrot, ntr, m, niter, titer, seed, basis="fourier", lam
     (32, 4, 2, 2, 24, 41, 3, 2),     # m = 41: frame groups 32 + 9, pair groups 16 + 5
     (50, 4, 2, 2, 16, 4, 3, 2),      # n = 50: 2n = 100, narrow last tiles
     (45, 4, 2, 2, 16, 4, 3, 2),      # odd n
+    (96, 6, 3, 2, 12, 6, 3, 2),      # 2n = 192 = 3 * 64 (compile-time 3 * 2^k transforms); m = 6: 2 slices packed
+    (80, 5, 5, 2, 10, 3, 3, 2),      # 2n = 160: runtime plan; m = 3: 4 slices packed, 5 = 4 + 1
 ])
 def test_run_parity_small(gpu, n, nz, nzp, nrot, ntr, m, niter, titer):
     """Result, rho sequence, r, s, ||f(k+1)-f(k)||^2, ||f(k+1)||^2 and both objective
