@@ -79,7 +79,10 @@ int rtv_synchronize(rtv_solver *h);
 /* Sets the state (NULL f / psi / mu: zero), forms div(mu - rho psi) for lambda1 and
  * restarts the iteration counter and history; rtv_iterate then continues from it. */
 int rtv_set_state(rtv_solver *h, const float *f, const float *psi, const float *mu, float rho, float lambda1);
-/* Copies the state out (any pointer may be NULL; rho: 1 float). */
+/* Copies the state out (any pointer may be NULL; rho: 1 float).  The solver keeps
+ * chi = mu - rho psi in place of psi (the dual part of the inner steps), so psi is
+ * returned as (mu - chi) / rho, equal to the psi of the last closing phase to FP32
+ * rounding; a set_state / get_state round trip returns f and mu bitwise. */
 int rtv_get_state(rtv_solver *h, float *f, float *psi, float *mu, float *dchi, float *rho);
 
 /* ---- filtered back-projection and support mask: the reference's unused USFFT
