@@ -433,14 +433,16 @@ constexpr int GM_WYP = 17;     // row-tap entries per item (3 zeros, 9 taps, 5 z
 template <int NB, int PK = 1>  // 16-column blocks: pairs of this launch <= 8*NB; PK slices per pass
 __global__ void __launch_bounds__(GM_THREADS)
 k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const SampleGeom *__restrict__ geo,
-              const int *__restrict__ items, const TileEntry *__restrict__ tiles, const float2 *__restrict__ cf,
+              const int *__restrict__ items, const TileEntry *__restrict__ tiles,
+              const unsigned short *__restrict__ cells, const float2 *__restrict__ cf,
               int n, int nt, int MPt, int p0, int accum, float beta, int nentries, int nzb, int zg) {
     static_assert(NB == 1 || NB == 2, "one or two column blocks");
     static_assert(PK == 1 || (NB == 1 && (PK == 2 || PK == 4)), "slice packing needs one column block");
     constexpr int NC = 16 * NB, HALF = 8 * NB, NCP = NC + 1;
     constexpr int CW = NC / PK, HP = CW / 2;  // columns per packed slice; its pairs
     constexpr int CELLS = TH * TH;
-    constexpr int NLD = (HALF * CELLS + GM_THREADS - 1) / GM_THREADS;  // tile loads per thread and pass
+    constexpr int TPS = GM_THREADS / HALF;          // staging threads per pair slot
+    constexpr int NLD = (CELLS + TPS - 1) / TPS;    // tile loads per thread and pass
     const int MP = min(HP, MPt - p0);  // pairs of this launch
     // 4 spare cells: the last 4-cell step of row TH-1 may read up to 3 cells past
     // the region (their weights are zero; the spare cells are zeroed)
@@ -460,59 +462,61 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     const TileEntry te = tiles[e];
     const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
     const int cnt = te.count;
-    // per-thread tile byte offsets within one slice's spectra (all pairs of a slice
-    // span MP*4n^2*8 < 2^32 bytes); entries outside the region or of absent pairs read
-    // the slice's first element and are not stored (the absent pairs' columns of the
-    // tile are zeroed once), so the loads need no predication.  Entry q belongs to
-    // packed slice tsl(q) of the pass.
+    // Tile staging: thread tid serves pair slot sp = tid / TPS (pair sp % HP of packed
+    // slice sp / HP) and the cells k0 + TPS*q of the entry's cell list (the cells of
+    // the region some item's window covers, raster order; the other cells stay zero
+    // and meet zero taps only).  Byte offsets within one slice's spectra (all pairs of
+    // a slice span MP*4n^2*8 < 2^32 bytes); entries past the list or of absent pairs
+    // read the slice's first element and are not stored, so the loads need no
+    // predication.
+    const int sp = tid / TPS, k0 = tid % TPS;
+    const int spp = sp % HP, tsl = sp / HP;
+    const int tcol = tsl * CW + spp;  // tile column of the real part
+    const unsigned short *cl = cells + te.pad;
+    const int ncell = cl[0];
     unsigned tofs[NLD];
-    unsigned keep = 0;  // bit q: entry q is stored
+    unsigned rcp[(NLD + 1) / 2];      // the cells, two 16-bit indices per word
+    unsigned keep = 0;                // bit q: entry q is stored
+#pragma unroll
+    for (int q = 0; q < (NLD + 1) / 2; q++) rcp[q] = 0u;
 #pragma unroll
     for (int q = 0; q < NLD; q++) {
-        const int idx = tid + q * GM_THREADS;
-        const int sp = idx / CELLS, rc = idx - sp * CELLS;
-        const int p = sp % HP;
+        const int k = k0 + TPS * q;
+        const int rc = cl[1 + max(min(k, ncell - 1), 0)];
         const int r = rc / TH, c = rc - r * TH;
         int gy = ty0 + r;
         if (gy >= N) gy -= N;
         int gx = tx0 + c;
         if (gx >= N) gx -= N;
-        const bool ok = idx < HALF * CELLS && p < MP;
-        tofs[q] = ok ? 8u * (unsigned)((p * N + gy) * N + gx) : 0u;
+        const bool ok = k < ncell && spp < MP;
+        tofs[q] = ok ? 8u * (unsigned)((spp * N + gy) * N + gx) : 0u;
         keep |= ok ? (1u << q) : 0u;
+        rcp[q / 2] |= (unsigned)rc << (16 * (q & 1));
     }
-    auto tsl = [&](int q) { return PK == 1 ? 0 : (tid + q * GM_THREADS) / CELLS / HP; };
+    // loads per thread that meet the list (equal across the workgroup)
+    const int nq = __builtin_amdgcn_readfirstlane((ncell + TPS - 1) / TPS);
     float2 stv[NLD];
     // the tile of the pass starting at slice z, of which nv slices exist (the others
     // read the last one: finite values in columns whose results are not written)
     auto load_tile = [&](int z, int nv) {
-        const char *Gz = reinterpret_cast<const char *>(G + ((size_t)z * MPt + p0) * N * N);
-        const size_t sb = (size_t)MPt * N * N * sizeof(float2);
+        const char *Gz = reinterpret_cast<const char *>(G + ((size_t)z * MPt + p0) * N * N) +
+                         (PK == 1 ? 0 : (size_t)min(tsl, nv - 1) * ((size_t)MPt * N * N * sizeof(float2)));
 #pragma unroll
         for (int q = 0; q < NLD; q++)
-            stv[q] = *reinterpret_cast<const float2 *>(Gz + (PK == 1 ? 0 : (size_t)min(tsl(q), nv - 1) * sb) + tofs[q]);
+            if (q < nq) stv[q] = *reinterpret_cast<const float2 *>(Gz + tofs[q]);
     };
     auto store_tile = [&] {
 #pragma unroll
         for (int q = 0; q < NLD; q++) {
             if (keep & (1u << q)) {
-                const int idx = tid + q * GM_THREADS;
-                const int sp = idx / CELLS, rc = idx - sp * CELLS;
-                const int col = (sp / HP) * CW + sp % HP;
-                tile[rc * NCP + col] = stv[q].x;
-                tile[rc * NCP + HP + col] = stv[q].y;
+                const int rc = (rcp[q / 2] >> (16 * (q & 1))) & 0xffff;
+                tile[rc * NCP + tcol] = stv[q].x;
+                tile[rc * NCP + HP + tcol] = stv[q].y;
             }
         }
     };
-    // columns of absent pairs (MP < HP) stay zero
-    for (int i = tid; i < CELLS * HALF; i += GM_THREADS) {
-        const int rc = i / HALF, sp = i - rc * HALF;
-        if (sp % HP >= MP) {
-            const int col = (sp / HP) * CW + sp % HP;
-            tile[rc * NCP + col] = 0.f;
-            tile[rc * NCP + HP + col] = 0.f;
-        }
-    }
+    // cells outside the list, columns of absent pairs (MP < HP) and the spare cells stay zero
+    for (int i = tid; i < (CELLS + 4) * NCP; i += GM_THREADS) tile[i] = 0.f;
     load_tile(z0, min(PK, zc));
     // stage the items: 4 threads per item (rows taps 0-4 / 5-8, cell taps 0-4 / 5-8)
     {
@@ -540,7 +544,6 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
             for (int a = 0; a < 3; a++) wty[it][a] = 0.f;
         if (part == 1)
             for (int a = 3 + KW; a < GM_WYP; a++) wty[it][a] = 0.f;
-        if (tid < 4 * NCP) tile[CELLS * NCP + tid] = 0.f;
     }
     lds_barrier();
     const int w = tid >> 6, l = tid & 63;

@@ -112,6 +112,7 @@ static bool make_plan(int N, FftPlan &p) {
 struct HostGeom {
     std::vector<SampleGeom> geo;
     std::vector<int> fitems;  // forward items per origin tile for the matrix-core gather (-1 = padding)
+    std::vector<unsigned short> fcells;  // per gather entry: its covered cells (count first)
     std::vector<int> asids;
     std::vector<AdjItem> aitems;  // asids with their window geometry
     std::vector<int2> arec;       // per adjoint record: {2 * sample | mirror, first coefficient of its row}
@@ -315,6 +316,29 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             for (int s0 = e0; s0 < e1; s0 += GM_ITEMS)
                 hg.fmtiles.push_back(TileEntry{t, s0, std::min(GM_ITEMS, e1 - s0), 0});
 #endif
+        }
+        // per entry, the cells of its TH x TH region that some item's 9 x 9 window
+        // covers, in raster order (about half of the region at the benchmark
+        // geometry): fcells[pad] = count, the cell indices r * TH + c follow.  The
+        // gather stages only these; the others stay zero and meet zero taps only.
+        hg.fcells.clear();
+        bool cov[TH * TH];
+        for (TileEntry &te : hg.fmtiles) {
+            const int ty0 = (te.tile / nT) * TS, tx0 = (te.tile % nT) * TS;
+            std::fill(cov, cov + TH * TH, false);
+            for (int i = te.start; i < te.start + te.count; i++) {
+                if (hg.fitems[i] < 0) continue;
+                const SampleGeom &g = hg.geo[hg.fitems[i]];
+                const int ly0 = (g.row0 - ty0 + N) % N, lx0 = (g.col0 - tx0 + N) % N;
+                if (ly0 >= TS || lx0 >= TS) throw Error(-1, "internal: gather item outside its origin tile");
+                for (int a = 0; a < KW; a++)
+                    for (int b = 0; b < KW; b++) cov[(ly0 + a) * TH + lx0 + b] = true;
+            }
+            te.pad = (int)hg.fcells.size();
+            hg.fcells.push_back(0);
+            for (int c = 0; c < TH * TH; c++)
+                if (cov[c]) hg.fcells.push_back((unsigned short)c);
+            hg.fcells[te.pad] = (unsigned short)(hg.fcells.size() - te.pad - 1);
         }
     }
     // adjoint lists: every tile the 9x9 window intersects, each tile's items
@@ -671,6 +695,7 @@ struct Solver {
     bool inputs = false;
     DBuf<SampleGeom> d_geo;
     DBuf<int> d_fitems;
+    DBuf<unsigned short> d_fcells;
     DBuf<AdjItem> d_aitems;
     DBuf<int2> d_arec;
     DBuf<TileEntry> d_fmtiles, d_atiles;
@@ -723,10 +748,11 @@ struct Solver {
         // geo 16 B, gather slots <= 16 ints, adjoint lists <= 4 records of 16 + 8 B and
         // block lists <= 9 bytes per item (16 B spare); per adjoint chunk its range, 16
         // block ranges, up to 3 padding bytes per block, its record range and SM_CH weight
-        // rows of 128 B; tables per ray / pixel and per row of the oversampled grid
+        // rows of 128 B; the gather entries' cell lists (at most ntile + ni / 16 entries of
+        // at most TH * TH + 1 shorts); tables per ray / pixel and per row of the oversampled grid
         const size_t nchk = ntile + 4 * ni / SM_CH + 1;
         const size_t geom = ni * (16 + 16 + 64 + 32 + 64 + 9) + nchk * (8 + 16 * 8 + 16 * 3 + 16 + SM_CH * 128) +
-                            3 * ntile * 16 +
+                            3 * ntile * 16 + (ntile + ni / 16 + 1) * (TH * TH + 1) * 2 +
                             (size_t)nt * 2 * MP * 8 + 8 * (size_t)n * 8 + 2 * (N + n) * 8 + N * 8;
         // partial sums of the prox (per workgroup) and of the ray passes (per ray, at most)
         const size_t pb = (size_t)((n + 1 + PX_ROWS - 1) / PX_ROWS) * (M + 1) * (nzp + 1) * nch;
@@ -958,6 +984,7 @@ struct Solver {
         beta = hg.beta;
         upload(d_geo, hg.geo, bytes, st);
         upload(d_fitems, hg.fitems, bytes, st);
+        upload(d_fcells, hg.fcells, bytes, st);
         upload(d_fmtiles, hg.fmtiles, bytes, st);
         nfm_entries = (int)hg.fmtiles.size();
         if (std::getenv("RECTV_VERBOSE")) {
@@ -1072,7 +1099,7 @@ struct Solver {
             launch("gather", [&] {
 #define RTV_GATHER(NB_, PK_)                                                                                          \
     hipLaunchKernelGGL((k_gather_mfma<NB_, PK_>), dim3(blocks), dim3(GM_THREADS), 0, st, d_G.p, d_gq.p, d_geo.p,      \
-                       d_fitems.p, d_fmtiles.p, d_cf.p, n, nt, MP, p0, acc, beta, nfm_entries, zc, zg)
+                       d_fitems.p, d_fmtiles.p, d_fcells.p, d_cf.p, n, nt, MP, p0, acc, beta, nfm_entries, zc, zg)
                 if (pk == 4) RTV_GATHER(1, 4);
                 else if (pk == 2) RTV_GATHER(1, 2);
                 else if (np <= 8) RTV_GATHER(1, 1);
@@ -1913,6 +1940,7 @@ struct Solver {
         d_aitems.release(t);
         d_arec.release(t);
         d_fitems.release(t);
+        d_fcells.release(t);
         d_fmtiles.release(t);
         d_atiles.release(t);
         d_runs.release(t);
