@@ -1321,6 +1321,7 @@ struct TvArgs {
     const float *rho;  // device-resident penalty (updated by k_rho_rule between outer iterations)
     float lam0;
     const int2 *supp = nullptr;  // support mask (kept x range [lo, hi) per row y), null: none
+    int zseg = 1;                // slice segments per chunk of the column-marching kernels
 };
 
 
@@ -1335,12 +1336,18 @@ constexpr int TV_ROWS = RTV_TV_ROWS;
 constexpr int TV_THREADS = 64 * TV_ROWS;
 struct TvTile {
     int x, y, c, b;  // column, row, chunk, logical workgroup index
+    int z0, z1;      // the chunk's slices [z0, z1) this column marches over
 };
-__device__ __forceinline__ TvTile tv_tile(int nx, int ny, int chunk0) {
-    const int gx = (nx + 63) / 64, gy = (ny + TV_ROWS - 1) / TV_ROWS;
+// Column-marching kernels split each chunk's slices into a.zseg segments (more
+// workgroups when n^2 x chunks alone would not fill the device); the grid holds
+// (x tiles, y tiles, segments, chunks) with x tiles fastest.
+__device__ __forceinline__ TvTile tv_tile(const TvArgs &a) {
+    const int gx = (a.n + 63) / 64, gy = (a.n + TV_ROWS - 1) / TV_ROWS;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int bx = L % gx, by = (L / gx) % gy, bz = L / (gx * gy);
-    return TvTile{bx * 64 + (int)(threadIdx.x & 63), by * TV_ROWS + (int)(threadIdx.x >> 6), chunk0 + bz, L};
+    const int sg = bz % a.zseg;
+    return TvTile{bx * 64 + (int)(threadIdx.x & 63), by * TV_ROWS + (int)(threadIdx.x >> 6), a.chunk0 + bz / a.zseg, L,
+                  sg * a.nzp / a.zseg, (sg + 1) * a.nzp / a.zseg};
 }
 
 __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i, int y, int x) {
@@ -1364,7 +1371,7 @@ __device__ __forceinline__ size_t dual_idx(const TvArgs &a, int c, int zl, int i
 template <int MM, bool CHI>
 __global__ void __launch_bounds__(TV_THREADS)
 k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *__restrict__ dchi, TvArgs a, int i0) {
-    const TvTile tt = tv_tile(a.n, a.n, a.chunk0);
+    const TvTile tt = tv_tile(a);
     const int x = tt.x, y = tt.y, c = tt.c;
     const int n = a.n, M = a.M, nzp = a.nzp;
     if (x >= n || y >= n) return;
@@ -1398,11 +1405,11 @@ k_divchi(const float4 *__restrict__ psi, const float4 *__restrict__ mu, float *_
         }
         return reinterpret_cast<const float *>(mu + d)[k] - r * *pp;
     };
-    float czc[MM];  // chi_z of the previous slice plane (ghost plane z' = 0 first)
+    float czc[MM];  // chi_z of the previous slice plane (the ghost plane z' = 0 for the chunk's first slice)
 #pragma unroll
     for (int i = 0; i < MM; i++)
-        if (i < Mg) czc[i] = chi(dual_idx(a, c, 0, i + 1, y + 1, x + 1)).w;
-    for (int zl = 0; zl < nzp; zl++) {
+        if (i < Mg) czc[i] = chi(dual_idx(a, c, tt.z0, i + 1, y + 1, x + 1)).w;
+    for (int zl = tt.z0; zl < tt.z1; zl++) {
         float ctc = chi(dual_idx(a, c, zl + 1, 0, y + 1, x + 1)).z;  // frame i0 - 1 (the ghost frame for i0 = 0)
         float *o = dchi + ((size_t)c * nzp + zl) * S + col;
 #pragma unroll
@@ -1436,7 +1443,7 @@ template <int MM>
 __global__ void __launch_bounds__(TV_THREADS)
 k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__restrict__ out,
          const float *__restrict__ dchi, const float2 *__restrict__ v, TvArgs a, int zv0, int i0) {
-    const TvTile tt = tv_tile(a.n, a.n, a.chunk0);
+    const TvTile tt = tv_tile(a);
     const int x = tt.x, y = tt.y, c = tt.c;
     const int n = a.n, M = a.M, nzp = a.nzp;
     if (x >= n || y >= n) return;
@@ -1453,10 +1460,11 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
     const float rgs = r * a.gs, rgt = r * a.gt;
     const float keep = in_support(a.supp, x, y) ? 1.f : 0.f;  // projection onto the support
     float fz[MM], fzp[MM], hzc[MM];
-    // own column at slice 0 and the z-gradient values on plane 0
+    // own column at the segment's first slice z0 and the z-gradient values on its
+    // plane (against slice z0 - 1; for z0 = 0 the left halo, or zero at the border)
     {
-        const auto rf = buf_rsrc(fm + zb * S);
-        const auto ra = buf_rsrc(flgl ? A + (zb - 1) * S : fm + zb * S);
+        const auto rf = buf_rsrc(fm + (zb + tt.z0) * S);
+        const auto ra = buf_rsrc(tt.z0 > 0 ? fm + (zb + tt.z0 - 1) * S : flgl ? A + (zb - 1) * S : fm + zb * S);
 #pragma unroll
         for (int i = 0; i < MM; i++) {
             if (i0 + i < M) {
@@ -1465,7 +1473,7 @@ k_update(const float *__restrict__ fm, const float *__restrict__ A, float *__res
             }
         }
     }
-    for (int zl = 0; zl < nzp; zl++) {
+    for (int zl = tt.z0; zl < tt.z1; zl++) {
         const size_t zg = zb + zl;
         {
             const auto rn = buf_rsrc(zl + 1 < nzp ? fm + (zg + 1) * S : (flgr ? A + (zg + 1) * S : fm + zg * S));

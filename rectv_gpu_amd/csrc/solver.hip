@@ -1176,6 +1176,14 @@ struct Solver {
 
     // workgroups of the TV kernels over an m x m (x, y) plane per chunk, for nc chunks
     static int tv_blocks(int m, int nc) { return ((m + 63) / 64) * ((m + TV_ROWS - 1) / TV_ROWS) * nc; }
+    // slice segments per chunk for the column-marching kernels (k_update, k_divchi):
+    // where the (x, y) tiles of the chunks give fewer workgroups than the device holds
+    // at once (two per CU), as many segments as still fit in that one round, of at
+    // least 2 slices each (each segment reads the slice before it once more)
+    static int tv_zseg(int m, int nc, int nzp) {
+        const int b = tv_blocks(m, nc);
+        return std::max(1, std::min(nzp / 2, 2 * 256 / b));
+    }
 
     TvArgs tv_args(float lam1, float lam0) const {
         TvArgs a;
@@ -1197,10 +1205,12 @@ struct Solver {
 
     template <int MM>
     void launch_update(const float *cur, const float *A, float *nxt, const TvArgs &ta, int ncb, int zv0) {
-        const dim3 grid(tv_blocks(n, ncb));
+        TvArgs tb = ta;
+        tb.zseg = tv_zseg(n, ncb, nzp);
+        const dim3 grid(tv_blocks(n, ncb) * tb.zseg);
         for (int i0 = 0; i0 < M; i0 += MM)  // groups of MM frames (one group unless M > 32)
             launch("update", [&] {
-                hipLaunchKernelGGL(k_update<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, nxt, v_dchi, d_v.p, ta, zv0,
+                hipLaunchKernelGGL(k_update<MM>, grid, dim3(TV_THREADS), 0, st, cur, A, nxt, v_dchi, d_v.p, tb, zv0,
                                    i0);
             });
     }
@@ -1233,9 +1243,10 @@ struct Solver {
     }
     template <int MM>
     void launch_divchi(const TvArgs &ta) {
-        const dim3 grid(tv_blocks(n, c_n));
         TvArgs tb = ta;
         tb.chunk0 = c_lo;
+        tb.zseg = tv_zseg(n, c_n, nzp);
+        const dim3 grid(tv_blocks(n, c_n) * tb.zseg);
         for (int i0 = 0; i0 < M; i0 += MM)
             launch("divchi", [&] {
                 hipLaunchKernelGGL((k_divchi<MM, true>), grid, dim3(TV_THREADS), 0, st, v_psi, v_mu, v_dchi, tb, i0);
