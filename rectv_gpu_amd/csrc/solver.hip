@@ -211,10 +211,21 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             keys.push_back(Key{((unsigned long long)ftile[id] << 32) | (unsigned)(g.row0 * N + g.col0), (int)id});
         }
         std::sort(keys.begin(), keys.end());
-        // flat lists for the matrix-core gather: items in origin raster order,
-        // packed into 16-item blocks (one wave each); a block is closed early
-        // when the next origin run would widen its window beyond 12 rows or 4
-        // steps of 4 cells; entries hold at most GM_ITEMS items
+        // flat lists for the matrix-core gather: the items of each origin tile packed
+        // into 16-item blocks (one wave each) spanning at most 12 rows and 4 steps of
+        // 4 cells, whose matrix-core work is rows x steps instructions.  Origins are
+        // taken by item count (most first, then raster order); an origin's items
+        // join the open block whose work grows least, if that growth per item stays
+        // below 0.6 of a lone origin's (a 9-row, 3-step window over 16 items), else
+        // they open a block.  Entries hold at most GM_ITEMS items.
+        struct GBlock {
+            int rlo, rhi, clo, chi, cnt;
+            int ids[16];
+        };
+        auto gcost = [](int rlo, int rhi, int clo, int chi) {
+            const int rows = rhi - rlo + KW, steps = (chi + KW - 1 - (clo & ~3)) / 4 + 1;
+            return (rows > 12 || steps > GM_KS) ? -1 : rows * steps;
+        };
         hg.fitems.clear();
         hg.fmtiles.clear();
         size_t a = 0;
@@ -222,39 +233,55 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             const int t = (int)(keys[a].k >> 32);
             const int ty0 = (t / nT) * TS, tx0 = (t % nT) * TS;
             const int e0 = (int)hg.fitems.size();
-            int bcnt = 0, rlo = 0, rhi = 0, clo = 0, chi = 0;
-            auto close_block = [&] {
-                while (bcnt > 0 && bcnt < 16) {
-                    hg.fitems.push_back(-1);
-                    bcnt++;
-                }
-                bcnt = 0;
+            struct Origin {
+                int ly0, lx0;
+                size_t a, b;  // its items keys[a .. b), ids ascending
             };
+            std::vector<Origin> org;
             while (a < keys.size() && (int)(keys[a].k >> 32) == t) {
                 size_t b = a;
                 while (b < keys.size() && keys[b].k == keys[a].k) b++;
                 const SampleGeom &g = hg.geo[keys[a].id];
-                const int ly0 = (g.row0 - ty0 + N) % N, lx0 = (g.col0 - tx0 + N) % N;
-                for (size_t c = a; c < b; c++) {
-                    if (bcnt > 0) {
-                        const int nrlo = std::min(rlo, ly0), nrhi = std::max(rhi, ly0);
-                        const int nclo = std::min(clo, lx0) & ~3, nchi = std::max(chi, lx0) + KW - 1;
-                        if (nrhi - nrlo + KW > 12 || (nchi - nclo) / 4 + 1 > 4) close_block();
-                    }
-                    if (bcnt == 0) {
-                        rlo = rhi = ly0;
-                        clo = chi = lx0;
-                    }
-                    rlo = std::min(rlo, ly0);
-                    rhi = std::max(rhi, ly0);
-                    clo = std::min(clo, lx0);
-                    chi = std::max(chi, lx0);
-                    hg.fitems.push_back(keys[c].id);
-                    if (++bcnt == 16) bcnt = 0;
-                }
+                org.push_back(Origin{(g.row0 - ty0 + N) % N, (g.col0 - tx0 + N) % N, a, b});
                 a = b;
             }
-            close_block();
+            std::stable_sort(org.begin(), org.end(), [](const Origin &p, const Origin &q) { return p.b - p.a > q.b - q.a; });
+            std::vector<GBlock> blocks;
+            for (const Origin &o : org) {
+                size_t c = o.a;
+                while (c < o.b) {
+                    const int left = (int)(o.b - c);
+                    int best = -1, binc = 0;
+                    for (int q = 0; q < (int)blocks.size(); q++) {
+                        const GBlock &bk = blocks[q];
+                        if (bk.cnt == 16) continue;
+                        const int c1 = gcost(std::min(bk.rlo, o.ly0), std::max(bk.rhi, o.ly0), std::min(bk.clo, o.lx0),
+                                             std::max(bk.chi, o.lx0));
+                        if (c1 < 0) continue;
+                        const int inc = c1 - gcost(bk.rlo, bk.rhi, bk.clo, bk.chi);
+                        if (best < 0 || inc < binc) {
+                            best = q;
+                            binc = inc;
+                        }
+                    }
+                    // join when inc / take < 0.6 * 27 / 16 (integer form)
+                    const int take_b = best >= 0 ? std::min(left, 16 - blocks[best].cnt) : 0;
+                    if (best < 0 || binc * 16 * 10 >= 27 * 6 * take_b) {
+                        blocks.push_back(GBlock{o.ly0, o.ly0, o.lx0, o.lx0, 0, {}});
+                        best = (int)blocks.size() - 1;
+                    }
+                    GBlock &bk = blocks[best];
+                    bk.rlo = std::min(bk.rlo, o.ly0);
+                    bk.rhi = std::max(bk.rhi, o.ly0);
+                    bk.clo = std::min(bk.clo, o.lx0);
+                    bk.chi = std::max(bk.chi, o.lx0);
+                    const int take = std::min(left, 16 - bk.cnt);
+                    for (int i = 0; i < take; i++) bk.ids[bk.cnt++] = keys[c + i].id;
+                    c += take;
+                }
+            }
+            for (const GBlock &bk : blocks)
+                for (int i = 0; i < 16; i++) hg.fitems.push_back(i < bk.cnt ? bk.ids[i] : -1);
             const int e1 = (int)hg.fitems.size();
             // the kernel relies on every block spanning at most 12 rows and 4 steps of 4 cells;
             // a block's matrix-core work is rows x steps instructions per column block
