@@ -1577,46 +1577,58 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
     const size_t sc0 = slab(zq, hc) * S, sn = slab(zq + 1, hn) * S;
     const int ic = iq > 0 ? iq - 1 : 0, it = iq < M ? iq : M - 1;
     const size_t fc = (size_t)ic * plane, ft = (size_t)it * plane;
+    // the three planes of extension values an entry reads (own, +t, +z) as buffer
+    // descriptors: uniform bases, 32-bit lane offsets
+    const auto rFc = buf_rsrc(fm + sc0 + fc), rAc = buf_rsrc(A + sc0 + fc);
+    const auto rFt = buf_rsrc(fm + sc0 + ft), rAt = buf_rsrc(A + sc0 + ft);
+    const auto rFn = buf_rsrc(fm + sn + fc), rAn = buf_rsrc(A + sn + fc);
+    float4 *const mup = mu + dual_idx(a, c, zq, iq, 0, 0);
+    float4 *const psp = psi + dual_idx(a, c, zq, iq, 0, 0);
+    const float th2 = th * th;
+    // per-entry terms are formed in FP32 and summed in FP64
     double sr = 0.0, ss = 0.0, sc = 0.0, sf = 0.0, st = 0.0;
     const int y1 = min(n + 1, (band + 1) * PX_ROWS);
     for (int yq = band * PX_ROWS + w; yq < y1; yq += PX_WAVES) {
         const int yc = yq > 0 ? yq - 1 : 0, yn = yq < n ? yq : n - 1;
         for (int xq = lane; xq <= n; xq += 64) {
             const int xc = xq > 0 ? xq - 1 : 0, xn = xq < n ? xq : n - 1;
-            const size_t o0 = (size_t)yc * n + xc, ox = (size_t)yc * n + xn, oy = (size_t)yn * n + xc;
+            const unsigned o0 = 4u * (unsigned)(yc * n + xc), ox = 4u * (unsigned)(yc * n + xn),
+                           oy = 4u * (unsigned)(yn * n + xc);
             // extension values (F) and the change against the previous iterate (D)
-            auto FD = [&](size_t o, bool h, float &F, float &D) {
-                const float av = A[o], fv = fm[o];
+            auto FD = [&](__amdgpu_buffer_rsrc_t rf, __amdgpu_buffer_rsrc_t ra, unsigned o, bool h, float &F,
+                          float &D) {
+                const float av = buf_ld(ra, o, 0), fv = buf_ld(rf, o, 0);
                 F = h ? av : fv;
                 D = h ? 0.f : av - fv;
             };
             float e0, d0, xF, xD, yF, yD, tF, tD, zF, zD;
-            FD(sc0 + fc + o0, hc, e0, d0);
-            FD(sc0 + fc + ox, hc, xF, xD);
-            FD(sc0 + fc + oy, hc, yF, yD);
-            FD(sc0 + ft + o0, hc, tF, tD);
-            FD(sn + fc + o0, hn, zF, zD);
+            FD(rFc, rAc, o0, hc, e0, d0);
+            FD(rFc, rAc, ox, hc, xF, xD);
+            FD(rFc, rAc, oy, hc, yF, yD);
+            FD(rFt, rAt, o0, hc, tF, tD);
+            FD(rFn, rAn, o0, hn, zF, zD);
             const float hx = a.gs * (xF - e0), hy = a.gs * (yF - e0);
             const float ht = a.gt * ((iq < M ? tF : e0) - e0), hz = a.gs * (zF - e0);
             const float dx = a.gs * (xD - d0), dy = a.gs * (yD - d0);
             const float dt = a.gt * ((iq < M ? tD : d0) - d0), dz = a.gs * (zD - d0);
-            ss += (double)dx * dx + (double)dy * dy + (double)dt * dt + (double)dz * dz;
-            st += (double)sqrtf(hx * hx + hy * hy + ht * ht + hz * hz);
+            ss += (double)(dx * dx + dy * dy + dt * dt + dz * dz);
+            st += (double)__builtin_amdgcn_sqrtf(hx * hx + hy * hy + ht * ht + hz * hz);
             if (zq >= 1 && iq >= 1 && yq >= 1 && xq >= 1) {
-                sc += (double)d0 * d0;
-                sf += (double)e0 * e0;
+                sc += (double)(d0 * d0);
+                sf += (double)(e0 * e0);
             }
-            const size_t d = dual_idx(a, c, zq, iq, yq, xq);
+            const unsigned dl = (unsigned)(yq * (n + 1) + xq);
             // mu, psi: streamed (non-temporal), so the caches keep the extension values of
             // fm and A that neighbouring entries read again
-            const v4f mv = __builtin_nontemporal_load((const v4f *)(mu + d));
+            const v4f mv = __builtin_nontemporal_load((const v4f *)(mup + dl));
             float4 m4 = make_float4(mv[0], mv[1], mv[2], mv[3]);
             float4 p4 = make_float4(hx + m4.x * rinv, hy + m4.y * rinv, ht + m4.z * rinv, hz + m4.w * rinv);
-            const float za = sqrtf(p4.x * p4.x + p4.y * p4.y + p4.z * p4.z + p4.w * p4.w);
-            if (za <= th) {
+            // shrink: |p| <= th -> 0, else p (1 - th / |p|)
+            const float q2 = p4.x * p4.x + p4.y * p4.y + p4.z * p4.z + p4.w * p4.w;
+            if (q2 <= th2) {
                 p4 = make_float4(0.f, 0.f, 0.f, 0.f);
             } else {
-                const float sh = th / za;
+                const float sh = th * __builtin_amdgcn_rsqf(q2);
                 p4.x -= sh * p4.x;
                 p4.y -= sh * p4.y;
                 p4.z -= sh * p4.z;
@@ -1628,12 +1640,12 @@ k_prox(const float *__restrict__ fm, const float *__restrict__ A, float4 *__rest
             m4.w += r * hz - r * p4.w;
             if (CHI)  // chi = mu' - rho psi in place of psi: k_divchi then reads one dual array
                 __builtin_nontemporal_store((v4f){m4.x - r * p4.x, m4.y - r * p4.y, m4.z - r * p4.z, m4.w - r * p4.w},
-                                            (v4f *)(psi + d));
+                                            (v4f *)(psp + dl));
             else
-                __builtin_nontemporal_store((v4f){p4.x, p4.y, p4.z, p4.w}, (v4f *)(psi + d));
-            __builtin_nontemporal_store((v4f){m4.x, m4.y, m4.z, m4.w}, (v4f *)(mu + d));
+                __builtin_nontemporal_store((v4f){p4.x, p4.y, p4.z, p4.w}, (v4f *)(psp + dl));
+            __builtin_nontemporal_store((v4f){m4.x, m4.y, m4.z, m4.w}, (v4f *)(mup + dl));
             const float ex = hx - p4.x, ey = hy - p4.y, et = ht - p4.z, ez = hz - p4.w;
-            sr += (double)ex * ex + (double)ey * ey + (double)et * et + (double)ez * ez;
+            sr += (double)(ex * ex + ey * ey + et * et + ez * ez);
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
