@@ -466,9 +466,9 @@ k_gather_mfma(const float2 *__restrict__ G, float2 *__restrict__ gq, const Sampl
     // slice sp / HP) and the cells k0 + TPS*q of the entry's cell list (the cells of
     // the region some item's window covers, raster order; the other cells stay zero
     // and meet zero taps only).  Byte offsets within one slice's spectra (all pairs of
-    // a slice span MP*4n^2*8 < 2^32 bytes); entries past the list or of absent pairs
-    // read the slice's first element and are not stored, so the loads need no
-    // predication.
+    // a slice span MP*4n^2*8 < 2^32 bytes).  A thread's loads stop at the list's end
+    // (nq, equal across the workgroup); entries below that bound but past the list,
+    // or of absent pairs, read the slice's first element and are not stored.
     const int sp = tid / TPS, k0 = tid % TPS;
     const int spp = sp % HP, tsl = sp / HP;
     const int tcol = tsl * CW + spp;  // tile column of the real part
