@@ -192,8 +192,8 @@ static void build_geometry(HostGeom &hg, int n, int nt, int M, const float *thet
             ftile[2 * id + 1] = (m.row0 / TS) * nT + m.col0 / TS;
         }
     }
-    // Work lists.  Forward: the items of the tile holding their window origin, in
-    // origin raster order (ids ascending), packed into 16-item blocks and entries
+    // Work lists.  Forward: the items of the tile holding their window origin,
+    // grouped by origin (ids ascending), packed into 16-item blocks and entries
     // of at most GM_ITEMS items; adjoint: every tile the 9x9 window intersects
     // (one entry per tile, empty tiles included).
     struct Key {
