@@ -129,6 +129,10 @@ def _case(name):
         n, nt, m, nz = 40, 16, 8, 3
         th = np.sort(rng.uniform(0, 4 * np.pi, nt)).astype(np.float32)
         phi, center = B.takephi(nt, m), n / 2 - 0.5
+    elif name == "n64_dense_angles":       # 8 rotations x 64 projections: up to 32 items per window
+        n, nt, m, nz = 64, 512, 4, 1       # origin (split over blocks), tiles of many blocks and entries
+        th = B.angles(64, 8)
+        phi, center = B.takephi(nt, m), n / 2 + 0.5
     elif name == "n96_rp3":                # 2n = 192 = 3 * 64: compile-time length 3 * 2^k
         n, nt, m, nz = 96, 24, 5, 3
         th = B.angles(12, 2)
@@ -155,7 +159,7 @@ CASES = ["n64_rand", "n72_mixed_radix_m3", "n384_mixed_radix", "cfg_small", "n64
          "n56_radix7_small", "n64_m40_pair_groups", "n32_m65_pair_groups", "n36_narrow_tile",
          "n50_three_tiles", "n100_narrow_tile", "n44_radix11", "n34_radix17", "n136_radix17", "n45_odd",
          "n17_odd_prime", "n63_odd", "n48_m4_packed_partial", "n40_m8_packed_partial", "n96_rp3",
-         "n160_rp5", "n320_rp5", "n768_rp3"]
+         "n160_rp5", "n320_rp5", "n768_rp3", "n64_dense_angles"]
 
 
 @pytest.mark.parametrize("name", CASES)
